@@ -1,0 +1,88 @@
+"""Builds the native parts of disptools_amd in-tree.
+
+* ``libdisptools_hip.so`` -- the gfx950 kernels and the C ABI declared in
+  ``include/disptools_hip.h`` (hipcc);
+* ``_disptools*.so`` -- the CPython extension with the reference's method
+  table (``python_c_extension/_disptools.c:532-540``), linked against it.
+
+Usage: ``python -m disptools_amd.build`` (or ``build_all()``).
+"""
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("DISPTOOLS_OFFLOAD_ARCH", "gfx950")
+
+LIB_NAME = "libdisptools_hip.so"
+LIB_PATH = os.path.join(PKG, LIB_NAME)
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=%s" % ARCH,
+    # The reference is ISO C99 without contraction; keep every multiply and add
+    # separately rounded so the FP64 stencil is bit-exact and results are stable.
+    "-ffp-contract=off",
+    # Packed FP32 instructions are kept out of the generated code.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I", INCLUDE, "-I", CSRC,
+]
+
+HIP_SOURCES = ["dt_kernels.hip", "dt_runtime.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library(verbose=True):
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, src + ".o")
+        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH] + objs + ["-o", tmp], verbose)
+    os.replace(tmp, LIB_PATH)
+    for o in objs:
+        os.remove(o)
+    return LIB_PATH
+
+
+def extension_path():
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    return os.path.join(PKG, "_disptools" + suffix)
+
+
+def build_extension(verbose=True):
+    import numpy as np
+    src = os.path.join(CSRC, "_disptools.c")
+    out = extension_path()
+    tmp = out + ".tmp"
+    cmd = [
+        "gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-Wall",
+        "-I", sysconfig.get_paths()["include"], "-I", np.get_include(), "-I", INCLUDE,
+        src, "-o", tmp, "-L", PKG, "-ldisptools_hip", "-Wl,-rpath,$ORIGIN",
+    ]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(verbose=True):
+    build_library(verbose)
+    build_extension(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-q" not in sys.argv)

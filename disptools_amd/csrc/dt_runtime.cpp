@@ -1,0 +1,488 @@
+// dt_runtime.cpp -- host runtime and C ABI of the HIP displacement library.
+//
+// A solve keeps all of its state on the device.  The host enqueues chunks of
+// pass kernels and reads back only the 200-byte control block once per chunk
+// (two chunks in flight), so the line search and the stopping tests never
+// wait for the host; passes launched after the solve has stopped return at
+// once.  Reference behaviour: src/displacement_field_gradient.c:191-431,
+// src/displacement_field_greedy.c:85-247; the host-driven CUDA loop this
+// replaces is cuda/displacement_field_gradient.cu:164-468.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "disptools_hip.h"
+#include "dt_internal.hpp"
+#include "dt_kernels.hpp"
+
+// ---------------------------------------------------------------------------
+// Error state (same contract as src/disptools.c:8-43)
+// ---------------------------------------------------------------------------
+
+static disptools_error_state g_error = {false, -1, "", "", "", ""};
+
+static void set_error(int line, const char *file, const char *func, const char *msg)
+{
+    g_error.error = true;
+    g_error.line = line;
+    std::snprintf(g_error.file, sizeof g_error.file, "%s", file);
+    std::snprintf(g_error.function, sizeof g_error.function, "%s", func);
+    std::snprintf(g_error.message, sizeof g_error.message, "%s", msg);
+    g_error.trace[0] = '\0';
+}
+
+#define DT_FAIL(msg)                                         \
+    do {                                                     \
+        set_error(__LINE__, __FILE__, __func__, (msg));      \
+        return -1;                                           \
+    } while (0)
+
+#define DT_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            char m_[1024];                                                            \
+            std::snprintf(m_, sizeof m_, "%s: %s", #call, hipGetErrorString(e_));     \
+            set_error(__LINE__, __FILE__, __func__, m_);                              \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+extern "C" disptools_error_state *disptools_get_error(void) { return &g_error; }
+extern "C" bool disptools_has_error(void) { return g_error.error; }
+extern "C" void disptools_clear_error(void)
+{
+    g_error.error = false;
+    g_error.line = -1;
+    g_error.file[0] = g_error.function[0] = g_error.message[0] = g_error.trace[0] = '\0';
+}
+extern "C" int get_float_type_size(void) { return 32; }
+
+extern "C" bool set_device(const int id)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return false;
+    if (id < 0 || id >= count) return false;
+    return hipSetDevice(id) == hipSuccess;
+}
+
+namespace dt {
+
+// ---------------------------------------------------------------------------
+// Device arena: one grow-only allocation per device, reused across calls
+// ---------------------------------------------------------------------------
+
+struct Arena {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    Ctrl *host_ctrl = nullptr;   // pinned, 2 slots
+};
+
+static std::mutex g_lock;
+static std::vector<Arena> g_arenas;
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static int arena_get(size_t bytes, Arena **out)
+{
+    int dev = 0;
+    DT_HIP(hipGetDevice(&dev));
+    if ((int)g_arenas.size() <= dev) g_arenas.resize(dev + 1);
+    Arena &a = g_arenas[dev];
+    if (!a.host_ctrl) DT_HIP(hipHostMalloc((void **)&a.host_ctrl, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    if (a.bytes < bytes) {
+        if (a.ptr) DT_HIP(hipFree(a.ptr));
+        a.ptr = nullptr;
+        a.bytes = 0;
+        const size_t want = align_up(bytes, (size_t)1 << 21);
+        if (hipMalloc(&a.ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            a.ptr = nullptr;
+            DT_FAIL("cannot allocate device memory for the solver");
+        }
+        a.bytes = want;
+    }
+    *out = &a;
+    return 0;
+}
+
+// Carves aligned sub-buffers out of a base pointer.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    explicit Carver(void *b) : base((char *)b) {}
+    template <typename T>
+    T *take(size_t count)
+    {
+        off = align_up(off, 256);
+        T *p = (T *)(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+// Bytes of device scratch a solve needs besides the caller's buffers.
+static size_t scratch_bytes(int algorithm, long long n, int ntiles, long long trace_cap)
+{
+    size_t b = 0;
+    auto add = [&](size_t s) { b = align_up(b, 256) + s; };
+    const int extra_fields = algorithm == DT_GRADIENT ? NBUF - 1 : 1;
+    for (int i = 0; i < extra_fields; ++i) add(3 * (size_t)n * sizeof(float));
+    if (algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
+    add((size_t)ntiles * 4 * sizeof(double));
+    add(sizeof(unsigned int));
+    add(sizeof(Ctrl));
+    add((size_t)std::max<long long>(trace_cap, 1) * 4 * sizeof(double));
+    add(sizeof(Inst));
+    add((size_t)ntiles * sizeof(Tile));
+    return align_up(b, 256) + 256;
+}
+
+// Default z-chunk depth of a tile.
+static int default_tz(int nx, int ny, int nz)
+{
+    const long long cols = (long long)((nx + TILE_X - 1) / TILE_X) * ((ny + TILE_Y - 1) / TILE_Y);
+    // aim for >= ~1024 blocks while keeping chunks at least 8 planes deep
+    int tz = 32;
+    while (tz > 8 && cols * ((nz + tz - 1) / tz) < 1024) tz /= 2;
+    return std::min(tz, std::max(nz, 1));
+}
+
+// Tile table ordered so that block b (dispatched to XCD b % 8) walks a
+// contiguous range of tiles on each XCD, keeping halo re-reads in one L2.
+static void build_tiles(std::vector<Tile> &out, int inst, int ntiles)
+{
+    const int nx = 8;
+    const int per = (ntiles + nx - 1) / nx;
+    for (int k = 0; k < per; ++k)
+        for (int x = 0; x < nx; ++x) {
+            const int t = x * per + k;
+            if (t < ntiles) out.push_back(Tile{inst, t});
+        }
+}
+
+static int precision_from_env()
+{
+    const char *s = std::getenv("DISPTOOLS_STENCIL_PRECISION");
+    if (s && std::strcmp(s, "64") == 0) return 64;
+    return 32;
+}
+
+static int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                    const dt_params *p)
+{
+    if (algorithm != DT_GRADIENT && algorithm != DT_GREEDY) DT_FAIL("invalid algorithm");
+    if (nx < 3 || ny < 3 || nz < 3)
+        DT_FAIL("every image dimension must be at least 3 voxels (the stencils read 3 samples)");
+    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) DT_FAIL("image too large");
+    if (!(dx > 0.0f) || !(dy > 0.0f) || !(dz > 0.0f)) DT_FAIL("spacing must be positive");
+    if (!p) DT_FAIL("null parameters");
+    return 0;
+}
+
+// Core solve on device-resident buffers.
+static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float dy, float dz,
+                        const float *d_target, const uint8_t *d_mask, const dt_params *p, float *d_field,
+                        dt_solve_info *info, double *trace, long long trace_cap, hipStream_t st)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    const long long n = (long long)nx * ny * nz;
+    const int tz = default_tz(nx, ny, nz);
+    const int ntx = (nx + TILE_X - 1) / TILE_X, nty = (ny + TILE_Y - 1) / TILE_Y, ntz = (nz + tz - 1) / tz;
+    const int ntiles = ntx * nty * ntz;
+    if (trace_cap < 0) trace_cap = 0;
+
+    Arena *A = nullptr;
+    if (arena_get(scratch_bytes(algorithm, n, ntiles, trace_cap), &A)) return -1;
+    Carver cv(A->ptr);
+
+    Inst h{};
+    h.nx = nx; h.ny = ny; h.nz = nz;
+    h.ntx = ntx; h.nty = nty; h.ntz = ntz; h.tz = tz;
+    h.n = n;
+    h.ix = (float)(1.0 / (double)dx);
+    h.iy = (float)(1.0 / (double)dy);
+    h.iz = (float)(1.0 / (double)dz);
+    h.dx = dx; h.dy = dy; h.dz = dz;
+    h.p = *p;
+    h.target = d_target;
+    h.mask = d_mask;
+    h.buf[0] = d_field;
+    const int extra_fields = algorithm == DT_GRADIENT ? NBUF - 1 : 1;
+    for (int i = 1; i <= extra_fields; ++i) h.buf[i] = cv.take<float>(3 * (size_t)n);
+    for (int i = extra_fields + 1; i < NBUF; ++i) h.buf[i] = nullptr;
+    if (algorithm == DT_GREEDY) {
+        h.err[0] = cv.take<float>((size_t)n);
+        h.err[1] = cv.take<float>((size_t)n);
+    }
+    h.partials = cv.take<double>((size_t)ntiles * 4);
+    h.counter = cv.take<unsigned int>(1);
+    h.ctrl = cv.take<Ctrl>(1);
+    h.trace = cv.take<double>((size_t)std::max<long long>(trace_cap, 1) * 4);
+    h.trace_cap = trace ? trace_cap : 0;
+    if (!trace) h.trace = nullptr;
+    Inst *d_inst = cv.take<Inst>(1);
+    Tile *d_tiles = cv.take<Tile>((size_t)ntiles);
+
+    std::vector<Tile> tiles;
+    tiles.reserve(ntiles);
+    build_tiles(tiles, 0, ntiles);
+
+    Ctrl c{};
+    c.active = 1;
+    c.phase = PH_INIT;
+    c.mode = TRIAL_DIRECT;
+    c.src = 0;
+    c.cand = 0;
+    c.base = 0;
+    c.eta = p->eta;
+    c.out_trial = -1;
+    if (algorithm == DT_GRADIENT) {
+        c.out_next = 1;
+        c.eta_next = p->eta * p->alpha;   // first search step (:290-291)
+    } else {
+        c.out_next = -1;
+        c.step_on = 0;
+        c.g_new = 1;
+        c.e_cur = 1;
+        c.e_new = 0;
+    }
+    c.result_buf = 0;
+
+    DT_HIP(hipMemcpyAsync(d_inst, &h, sizeof(Inst), hipMemcpyHostToDevice, st));
+    DT_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
+    DT_HIP(hipMemcpyAsync(h.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+    DT_HIP(hipMemsetAsync(h.counter, 0, sizeof(unsigned int), st));
+    if (algorithm == DT_GREEDY) {
+        // the second field buffer starts at zero: its outer face is never written
+        // by the greedy step (src/disptools.c:61, displacement_field_greedy.c:43-65)
+        DT_HIP(hipMemsetAsync(h.buf[1], 0, 3 * (size_t)n * sizeof(float), st));
+    }
+
+    const int precision = precision_from_env();
+    hipEvent_t ev[2];
+    DT_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    DT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+
+    // passes needed in the worst case before the controller stops the solve
+    const long long per_iteration = 64;
+    const long long max_passes = 4096 + per_iteration * (std::max<long long>(p->it_max, 0) + 2);
+    long long launched = 0;
+    int rc = 0;
+    auto enqueue = [&](int k, int slot) -> int {
+        for (int i = 0; i < k; ++i) {
+            if (algorithm == DT_GRADIENT) {
+                DT_HIP(launch_grad_pass(precision, d_inst, d_tiles, ntiles, st));
+            } else {
+                DT_HIP(launch_greedy_step(d_inst, d_tiles, ntiles, st));
+                DT_HIP(launch_greedy_eval(precision, d_inst, d_tiles, ntiles, st));
+            }
+        }
+        launched += k;
+        DT_HIP(hipMemcpyAsync(&A->host_ctrl[slot], h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+        DT_HIP(hipEventRecord(ev[slot], st));
+        return 0;
+    };
+
+    int chunk = 4;
+    rc = enqueue(chunk, 0);
+    if (!rc) rc = enqueue(chunk, 1);
+    int slot = 0;
+    bool done = false;
+    while (!rc) {
+        if (hipEventSynchronize(ev[slot]) != hipSuccess) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
+        if (A->host_ctrl[slot].phase == PH_DONE) { done = true; break; }
+        if (launched > max_passes) {
+            set_error(__LINE__, __FILE__, __func__, "solver did not terminate (check alpha > 1 and beta < 1)");
+            rc = -1;
+            break;
+        }
+        chunk = std::min(chunk * 2, 64);
+        rc = enqueue(chunk, slot);
+        slot ^= 1;
+    }
+    hipError_t se = hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
+    if (rc) return -1;
+    (void)done;
+
+    Ctrl fin;
+    DT_HIP(hipMemcpy(&fin, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    if (fin.result_buf != 0) {
+        DT_HIP(hipMemcpyAsync(d_field, h.buf[fin.result_buf], 3 * (size_t)n * sizeof(float),
+                              hipMemcpyDeviceToDevice, st));
+    }
+    if (trace && trace_cap > 0) {
+        const long long nl = std::min(fin.trace_len, trace_cap);
+        if (nl > 0) DT_HIP(hipMemcpyAsync(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    DT_HIP(hipStreamSynchronize(st));
+    if (info) {
+        info->iterations = fin.it;
+        info->evaluations = fin.evaluations;
+        info->rejections = fin.rejections;
+        info->launches = launched;
+        info->reason = fin.reason;
+        info->final_error = fin.E_result;
+        info->final_max_error = fin.m_result;
+        info->trace_len = fin.trace_len;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    return 0;
+}
+
+}  // namespace dt
+
+using namespace dt;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int dt_solve_device(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                               const float *d_target, const unsigned char *d_mask, const dt_params *params,
+                               float *d_field, dt_solve_info *info, double *trace, long long trace_cap,
+                               void *stream)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    disptools_clear_error();
+    if (validate(algorithm, nx, ny, nz, dx, dy, dz, params)) return -1;
+    if (!d_target || !d_mask || !d_field) DT_FAIL("null buffer");
+    return solve_device(algorithm, (int)nx, (int)ny, (int)nz, dx, dy, dz, d_target, d_mask, params, d_field,
+                        info, trace, trace_cap, (hipStream_t)stream);
+}
+
+extern "C" int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                             const float *target, const bool *mask, const dt_params *params, float *field,
+                             dt_solve_info *info, double *trace, long long trace_cap)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    disptools_clear_error();
+    if (validate(algorithm, nx, ny, nz, dx, dy, dz, params)) return -1;
+    if (!target || !mask || !field) DT_FAIL("null buffer");
+    const size_t n = nx * ny * nz;
+    void *io = nullptr;
+    const size_t bytes = align_up(4 * n * sizeof(float), 256) + align_up(n, 256);
+    if (hipMalloc(&io, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        DT_FAIL("cannot allocate device memory for the inputs");
+    }
+    float *d_field = (float *)io;
+    float *d_target = d_field + 3 * n;
+    uint8_t *d_mask = (uint8_t *)((char *)io + align_up(4 * n * sizeof(float), 256));
+    int rc = 0;
+    hipStream_t st = nullptr;
+    if (hipMemcpy(d_field, field, 3 * n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_target, target, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_mask, mask, n, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(__LINE__, __FILE__, __func__, "host to device copy failed");
+        rc = -1;
+    }
+    if (!rc) rc = solve_device(algorithm, (int)nx, (int)ny, (int)nz, dx, dy, dz, d_target, d_mask, params,
+                               d_field, info, trace, trace_cap, st);
+    if (!rc && hipMemcpy(field, d_field, 3 * n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error(__LINE__, __FILE__, __func__, "device to host copy failed");
+        rc = -1;
+    }
+    (void)hipFree(io);
+    return rc;
+}
+
+static void reference_entry(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                            const float *J, const bool *mask, float epsilon, float tolerance, float eta,
+                            float eta_max, float alpha, float beta, float gamma, float delta, float zeta,
+                            float theta, float iota, bool strict, size_t it_max, float *field)
+{
+    dt_params p;
+    p.epsilon = epsilon; p.tolerance = tolerance; p.eta = eta; p.eta_max = eta_max;
+    p.alpha = alpha; p.beta = beta; p.gamma = gamma; p.delta = delta; p.zeta = zeta;
+    p.theta = theta; p.iota = iota; p.strict = strict ? 1 : 0;
+    p.it_max = (long long)it_max;
+    (void)dt_solve_host(algorithm, nx, ny, nz, dx, dy, dz, J, mask, &p, field, nullptr, nullptr, 0);
+}
+
+extern "C" void generate_displacement_gradient_cuda(
+    const size_t nx, const size_t ny, const size_t nz, const float dx, const float dy, const float dz,
+    const float *J, const bool *mask, const float epsilon, const float tolerance, float eta,
+    const float eta_max, const float alpha, const float beta, const float gamma, const float delta,
+    const float zeta, const float theta, const float iota, const bool strict, const size_t it_max,
+    float *field)
+{
+    reference_entry(DT_GRADIENT, nx, ny, nz, dx, dy, dz, J, mask, epsilon, tolerance, eta, eta_max, alpha,
+                    beta, gamma, delta, zeta, theta, iota, strict, it_max, field);
+}
+
+extern "C" void generate_displacement_greedy_cuda(
+    const size_t nx, const size_t ny, const size_t nz, const float dx, const float dy, const float dz,
+    const float *J, const bool *mask, const float epsilon, const float tolerance, float eta,
+    const float eta_max, const float alpha, const float beta, const float gamma, const float delta,
+    const float zeta, const float theta, const float iota, const bool strict, const size_t it_max,
+    float *field)
+{
+    reference_entry(DT_GREEDY, nx, ny, nz, dx, dy, dz, J, mask, epsilon, tolerance, eta, eta_max, alpha,
+                    beta, gamma, delta, zeta, theta, iota, strict, it_max, field);
+}
+
+extern "C" int dt_jacobian_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                                  const float *d_field, float *d_J, void *stream)
+{
+    if (nx < 3 || ny < 3 || nz < 3)
+        DT_FAIL("every image dimension must be at least 3 voxels (the stencils read 3 samples)");
+    if (!d_field || !d_J) DT_FAIL("null buffer");
+    const float ix = (float)(1.0 / (double)dx), iy = (float)(1.0 / (double)dy), iz = (float)(1.0 / (double)dz);
+    DT_HIP(launch_jacobian(d_field, d_J, (int)nx, (int)ny, (int)nz, ix, iy, iz, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" void jacobian_dynamic(const size_t nx, const size_t ny, const size_t nz, const float dx,
+                                 const float dy, const float dz, const float *f, float *J)
+{
+    disptools_clear_error();
+    if (nx < 3 || ny < 3 || nz < 3) {
+        set_error(__LINE__, __FILE__, __func__,
+                  "every image dimension must be at least 3 voxels (the stencils read 3 samples)");
+        return;
+    }
+    const size_t n = nx * ny * nz;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, 4 * n * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(__LINE__, __FILE__, __func__, "cannot allocate device memory");
+        return;
+    }
+    float *d_f = (float *)mem, *d_J = d_f + 3 * n;
+    if (hipMemcpy(d_f, f, 3 * n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        dt_jacobian_device(nx, ny, nz, dx, dy, dz, d_f, d_J, nullptr) != 0 ||
+        hipMemcpy(J, d_J, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        if (!g_error.error) set_error(__LINE__, __FILE__, __func__, "jacobian failed on the device");
+    }
+    (void)hipFree(mem);
+}
+
+extern "C" void regularise(const size_t nx, const size_t ny, const size_t nz, float *J, float epsilon)
+{
+    disptools_clear_error();
+    const size_t n = nx * ny * nz;
+    if (n == 0) return;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, n * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(__LINE__, __FILE__, __func__, "cannot allocate device memory");
+        return;
+    }
+    float *d_J = (float *)mem;
+    if (hipMemcpy(d_J, J, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        launch_regularise(d_J, (long long)n, epsilon, nullptr) != hipSuccess ||
+        hipMemcpy(J, d_J, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error(__LINE__, __FILE__, __func__, "regularise failed on the device");
+    }
+    (void)hipFree(mem);
+}

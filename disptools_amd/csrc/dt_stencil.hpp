@@ -1,0 +1,81 @@
+// dt_stencil.hpp -- finite-difference stencils of the Jacobian determinant.
+//
+// Arithmetic follows the reference (src/headers/jacobian_macros.h:4-148,
+// src/headers/disptools.h:186-199): sample differences are taken in float,
+// weights and the determinant in the compute type CT.  With CT = double and
+// contraction disabled the result is bit-identical to the reference C build;
+// CT = float is the fast solver path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace dt {
+
+// Stencil kinds along one axis.
+enum Kind : int { K_C4 = 0, K_C2 = 1, K_FWD = 2, K_BWD = 3 };
+
+// Stencil that the reference jacobian() leaves at a voxel (src/jacobian.c):
+// the outer face uses one-sided differences on the axes where it touches the
+// face and 2nd-order central elsewhere (:112-221); the penultimate layer uses
+// 2nd-order central on all axes (:223-241); everything else 4th-order central
+// (:87-110).
+__device__ __forceinline__ void classify(int x, int y, int z, int nx, int ny, int nz, int k[3])
+{
+    const bool ox = (x == 0) | (x == nx - 1);
+    const bool oy = (y == 0) | (y == ny - 1);
+    const bool oz = (z == 0) | (z == nz - 1);
+    if (ox | oy | oz) {
+        k[0] = x == 0 ? K_FWD : (x == nx - 1 ? K_BWD : K_C2);
+        k[1] = y == 0 ? K_FWD : (y == ny - 1 ? K_BWD : K_C2);
+        k[2] = z == 0 ? K_FWD : (z == nz - 1 ? K_BWD : K_C2);
+    } else {
+        const bool pen = (x == 1) | (x == nx - 2) | (y == 1) | (y == ny - 2) | (z == 1) | (z == nz - 2);
+        k[0] = k[1] = k[2] = pen ? K_C2 : K_C4;
+    }
+}
+
+// Derivative along one axis from the five samples f(-2) .. f(+2).
+template <typename CT>
+__device__ __forceinline__ CT dstencil(int kind, float m2, float m1, float c0, float p1, float p2, float inv)
+{
+    switch (kind) {
+    case K_C4:
+        return ((CT)(p1 - m1) * (CT)(2.0 / 3.0) - (CT)(p2 - m2) * (CT)(1.0 / 12.0)) * (CT)inv;
+    case K_C2:
+        return (CT)((p1 - m1) * inv) * (CT)0.5;
+    case K_FWD:
+        return ((CT)-1.5 * (CT)c0 + (CT)2.0 * (CT)p1 - (CT)0.5 * (CT)p2) * (CT)inv;
+    default:
+        return ((CT)1.5 * (CT)c0 - (CT)2.0 * (CT)m1 + (CT)0.5 * (CT)m2) * (CT)inv;
+    }
+}
+
+template <typename CT>
+__device__ __forceinline__ CT dstencil_c4(float m2, float m1, float p1, float p2, float inv)
+{
+    return ((CT)(p1 - m1) * (CT)(2.0 / 3.0) - (CT)(p2 - m2) * (CT)(1.0 / 12.0)) * (CT)inv;
+}
+
+// det(D + I), cofactor expansion along the first row (disptools.h:186-199).
+template <typename CT>
+__device__ __forceinline__ CT det_plus_identity(const CT d[3][3])
+{
+    const CT a11 = d[0][0] + (CT)1.0, a12 = d[0][1], a13 = d[0][2];
+    const CT a21 = d[1][0], a22 = d[1][1] + (CT)1.0, a23 = d[1][2];
+    const CT a31 = d[2][0], a32 = d[2][1], a33 = d[2][2] + (CT)1.0;
+    return a11 * (a22 * a33 - a23 * a32)
+         - a12 * (a21 * a33 - a23 * a31)
+         + a13 * (a21 * a32 - a22 * a31);
+}
+
+// Voxel error (src/headers/error.h:128-142): full error inside the mask,
+// soft-thresholded by `tol` outside it.
+__device__ __forceinline__ float voxel_error(float J, float T, bool in_mask, float tol)
+{
+    const float err = J - T;
+    if (in_mask) return err;
+    const float a = fabsf(err);
+    return a < tol ? 0.0f : copysignf(a - tol, err);
+}
+
+}  // namespace dt

@@ -1,0 +1,111 @@
+"""Synthetic inputs of the benchmark configurations (seeded, numpy PCG64).
+
+Volumes are numpy arrays indexed [z, y, x]; spacing tuples are (dx, dy, dz).
+
+* ``sphere_case``      -- 40^3 sphere: target 1.1 inside a ball of radius 10
+                          centred at n//2 and 1.0 outside, mask = the ball, as
+                          built by the reference's ``drawing.create_sphere``
+                          (drawing.py:123-242, Euclidean rho <= radius).
+* ``smooth_target``    -- exp(g), g i.i.d. N(0, sigma_g^2) smoothed by a Gaussian
+                          of ``smooth`` voxels (edge mode "nearest"), clipped.
+* ``ellipsoid_mask``   -- union of k ~ U{3..8} ellipsoids, semi-axes U[8, 30]
+                          voxels, centres U[0.2 n, 0.8 n], Haar-random rotations.
+* ``volume_128``       -- 128^3, spacing (1, 1, 1.5), target as above with
+                          sigma_g = 0.05, smoothing 4, clip [0.8, 1.25].
+* ``batch_uneven``     -- 64 volumes, each axis U{48..160}, spacing 1.
+* ``precision_256``    -- 256^3, spacing (0.8, 0.8, 1.0), sigma_g = 0.08,
+                          smoothing 6, clip [0.7, 1.4], two spheres of radii 40
+                          and 60 centred at (x, y, z) = (96, 128, 128), (160, 128, 128).
+* ``serving_stream``   -- Poisson arrivals with log-normal cube sides.
+"""
+
+import math
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+SEED_128 = 12345
+
+
+def sphere_case(n=40, radius=10.0, inside=1.1, outside=1.0):
+    c = n // 2
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    ball = r <= radius
+    target = np.where(ball, inside, outside).astype(np.float32)
+    return target, ball.copy(), (1.0, 1.0, 1.0)
+
+
+def smooth_target(shape, rng, sigma_g=0.05, smooth=4.0, clip=(0.8, 1.25)):
+    g = rng.normal(0.0, sigma_g, size=shape)
+    g = gaussian_filter(g, smooth, mode="nearest", truncate=4.0)
+    return np.clip(np.exp(g), clip[0], clip[1]).astype(np.float32)
+
+
+def _haar_rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoid_mask(shape, rng, k_range=(3, 8), axes_range=(8.0, 30.0)):
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pts = np.stack([x, y, z], axis=-1).astype(np.float64)
+    mask = np.zeros(shape, bool)
+    k = int(rng.integers(k_range[0], k_range[1] + 1))
+    n = np.array([nx, ny, nz], np.float64)
+    for _ in range(k):
+        semi = rng.uniform(axes_range[0], axes_range[1], size=3)
+        centre = rng.uniform(0.2 * n, 0.8 * n)
+        rot = _haar_rotation(rng)
+        local = (pts - centre) @ rot        # R^T (p - c) for row vectors
+        mask |= np.sum((local / semi) ** 2, axis=-1) <= 1.0
+    return mask
+
+
+def volume_case(shape, seed, sigma_g=0.05, smooth=4.0, clip=(0.8, 1.25)):
+    rng = np.random.default_rng(seed)
+    target = smooth_target(shape, rng, sigma_g, smooth, clip)
+    mask = ellipsoid_mask(shape, rng)
+    return target, mask
+
+
+def volume_128(seed=SEED_128):
+    target, mask = volume_case((128, 128, 128), seed)
+    return target, mask, (1.0, 1.0, 1.5)
+
+
+def batch_uneven(count=64, seed=2024, lo=48, hi=160):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for i in range(count):
+        shape = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
+        target, mask = volume_case(shape, seed * 1000 + i)
+        cases.append((target, mask, (1.0, 1.0, 1.0)))
+    return cases
+
+
+def precision_256():
+    rng = np.random.default_rng(256)
+    shape = (256, 256, 256)
+    target = smooth_target(shape, rng, sigma_g=0.08, smooth=6.0, clip=(0.7, 1.4))
+    z, y, x = np.meshgrid(np.arange(256), np.arange(256), np.arange(256), indexing="ij")
+    mask = ((x - 96) ** 2 + (y - 128) ** 2 + (z - 128) ** 2 <= 40 ** 2) | \
+           ((x - 160) ** 2 + (y - 128) ** 2 + (z - 128) ** 2 <= 60 ** 2)
+    return target, mask, (0.8, 0.8, 1.0)
+
+
+def serving_stream(count, qps, seed=7, median=96, sigma_ln=0.35, lo=32, hi=256, greedy_frac=0.2):
+    """Requests (arrival time, side, algorithm, seed) of a Poisson stream."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / qps)
+        side = int(np.clip(round(median * math.exp(rng.normal(0.0, sigma_ln))), lo, hi))
+        alg = "greedy" if rng.random() < greedy_frac else "gradient"
+        out.append(dict(arrival=t, side=side, algorithm=alg, seed=int(seed * 100000 + i)))
+    return out

@@ -1,0 +1,148 @@
+/*
+ * disptools_hip.h -- C ABI of the MI355X (gfx950) displacement-generation library.
+ *
+ * Plain C types only.  Every pointer argument of the reference-compatible
+ * entry points is caller-owned HOST memory, exactly as in the reference;
+ * the dt_*_device entry points take DEVICE pointers and a hipStream_t passed
+ * as `void *` (NULL = the default stream).
+ *
+ * FLOATING is float: the library is the FLOAT_SIZE=32 build of the reference
+ * ABI (reference CMakeLists.txt:106-111).
+ */
+#ifndef DISPTOOLS_HIP_H
+#define DISPTOOLS_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Reference-compatible entry points                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Error state; replaces src/headers/disptools.h:205-212 (same layout). */
+typedef struct disptools_error_state {
+    bool error;
+    int line;
+    char file[512];
+    char function[512];
+    char message[1024];
+    char trace[4096];
+} disptools_error_state;
+
+/* Replaces disptools_get_error / disptools_has_error / disptools_clear_error
+ * (src/headers/disptools.h:250,257,264; src/disptools.c:26-43). */
+disptools_error_state *disptools_get_error(void);
+bool disptools_has_error(void);
+void disptools_clear_error(void);
+
+/* Replaces get_float_type_size (src/headers/disptools.h:289): returns 32. */
+int get_float_type_size(void);
+
+/* Replaces set_device (cuda/headers/generate_displacement.cuh:11,
+ * cuda/device.cu:4-15): false if id is not a valid HIP device. */
+bool set_device(const int id);
+
+/* Replaces generate_displacement_gradient_cuda
+ * (cuda/headers/generate_displacement.cuh:39-62); the semantics are those of
+ * the reference C solver generate_displacement_gradient
+ * (src/displacement_field_gradient.c:191-431).  `field` is the initial guess
+ * on entry and the result on exit, layout [3][nz][ny][nx]. */
+void generate_displacement_gradient_cuda(
+        const size_t nx, const size_t ny, const size_t nz,
+        const float dx, const float dy, const float dz,
+        const float *J, const bool *mask,
+        const float epsilon, const float tolerance, float eta, const float eta_max,
+        const float alpha, const float beta, const float gamma, const float delta,
+        const float zeta, const float theta, const float iota,
+        const bool strict, const size_t it_max, float *field);
+
+/* Replaces generate_displacement_greedy_cuda
+ * (cuda/headers/generate_displacement.cuh:14-37) with the semantics of
+ * generate_displacement_greedy (src/displacement_field_greedy.c:85-247). */
+void generate_displacement_greedy_cuda(
+        const size_t nx, const size_t ny, const size_t nz,
+        const float dx, const float dy, const float dz,
+        const float *J, const bool *mask,
+        const float epsilon, const float tolerance, float eta, const float eta_max,
+        const float alpha, const float beta, const float gamma, const float delta,
+        const float zeta, const float theta, const float iota,
+        const bool strict, const size_t it_max, float *field);
+
+/* Replaces jacobian_dynamic (src/headers/jacobian.h:24-33, src/jacobian.c:7-21)
+ * as bound by _disptools.jacobian (python_c_extension/_disptools.c:238-274):
+ * f is [3][nz][ny][nx], J is [nz][ny][nx]; computed on the current device. */
+void jacobian_dynamic(const size_t nx, const size_t ny, const size_t nz,
+                      const float dx, const float dy, const float dz,
+                      const float *f, float *J);
+
+/* Replaces regularise (src/headers/jacobian.h:13-19, src/jacobian.c:29-55):
+ * J[i] = max-threshold at epsilon, in place, on the current device. */
+void regularise(const size_t nx, const size_t ny, const size_t nz, float *J, float epsilon);
+
+/* ------------------------------------------------------------------------ */
+/* Additive entry points (no counterpart in the reference)                 */
+/* ------------------------------------------------------------------------ */
+
+enum dt_algorithm { DT_GRADIENT = 0, DT_GREEDY = 1 };
+
+/* Termination reasons of a solve. */
+enum dt_reason {
+    DT_RUNNING = 0,
+    DT_STOP_ERROR_EXPLODED = 1,   /* !isnormal(total error)               */
+    DT_STOP_ETA_SMALL = 2,        /* eta < iota                           */
+    DT_STOP_NO_IMPROVEMENT = 3,   /* 1 - E / E_last < theta               */
+    DT_STOP_VOXEL_EXPLODED = 4,   /* !isnormal(max voxel error)           */
+    DT_STOP_TOLERANCE = 5,        /* max voxel error < epsilon            */
+    DT_STOP_MAX_ITERATIONS = 6    /* it_max iterations accepted           */
+};
+
+/* Solver parameters, same meaning as the reference arguments. */
+typedef struct dt_params {
+    float epsilon, tolerance, eta, eta_max, alpha, beta, gamma, delta, zeta, theta, iota;
+    int strict;
+    long long it_max;
+} dt_params;
+
+/* Record of one solve. */
+typedef struct dt_solve_info {
+    long long iterations;    /* iteration counter at exit                   */
+    long long evaluations;   /* stencil passes that evaluated a trial       */
+    long long rejections;    /* rejected trials                             */
+    long long launches;      /* pass kernels launched (incl. idle ones)     */
+    int reason;              /* enum dt_reason                              */
+    double final_error;      /* energy of the returned field (FP64 sum)     */
+    double final_max_error;  /* masked max |e| of the returned field        */
+    double seconds;          /* wall time of the solve on the device        */
+    long long trace_len;     /* log lines produced (it, E, max, eta each)   */
+} dt_solve_info;
+
+/* Jacobian of a device-resident field (device pointers). Returns 0 or -1. */
+int dt_jacobian_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                       const float *d_field, float *d_J, void *stream);
+
+/* Solve on device-resident data: d_target [nz][ny][nx], d_mask bytes,
+ * d_field [3][nz][ny][nx] in/out.  `trace` (host, optional) receives up to
+ * trace_cap log lines of 4 doubles (it, E, max, eta).  Returns 0 or -1
+ * (error state set).  Runs on `stream` and synchronises it at the end. */
+int dt_solve_device(int algorithm, size_t nx, size_t ny, size_t nz,
+                    float dx, float dy, float dz,
+                    const float *d_target, const unsigned char *d_mask,
+                    const dt_params *params, float *d_field,
+                    dt_solve_info *info, double *trace, long long trace_cap, void *stream);
+
+/* Same as dt_solve_device with host pointers (one H2D and one D2H copy). */
+int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
+                  float dx, float dy, float dz,
+                  const float *target, const bool *mask,
+                  const dt_params *params, float *field,
+                  dt_solve_info *info, double *trace, long long trace_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DISPTOOLS_HIP_H */
