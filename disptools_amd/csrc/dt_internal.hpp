@@ -64,6 +64,7 @@ struct Ctrl {
     long long evaluations;
     long long rejections;
     long long trace_len;
+    long long field_io;      // 3-component field volumes read + written by evaluated passes
 };
 
 // One solve instance (a volume with its parameters and buffers).

@@ -172,6 +172,8 @@ __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m, doubl
 {
     const dt_params &p = I.p;
     C->evaluations++;
+    // algorithmic traffic of the pass that was just evaluated
+    C->field_io += (C->mode == TRIAL_INTERP ? 2 : 1) + (C->out_trial >= 0 ? 1 : 0) + (C->out_next >= 0 ? 1 : 0);
     switch (C->phase) {
     case PH_INIT: {
         // trial = initial guess in buf[src]; its candidate was formed at eta * alpha
@@ -254,6 +256,8 @@ __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float m)
 {
     const dt_params &p = I.p;
     C->evaluations++;
+    // step: read u, write u; evaluation: read u (error maps are counted by the caller)
+    C->field_io += C->phase == PH_INIT ? 1 : 3;
     const int e_written = C->e_new;
     if (C->phase == PH_INIT) {
         C->E0 = E; C->m0 = m;

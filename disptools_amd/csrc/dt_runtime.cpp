@@ -334,6 +334,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         info->final_error = fin.E_result;
         info->final_max_error = fin.m_result;
         info->trace_len = fin.trace_len;
+        info->field_io = fin.field_io;
         info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     }
     return 0;

@@ -69,6 +69,7 @@ class SolveInfo(ctypes.Structure):
         ("final_max_error", ctypes.c_double),
         ("seconds", ctypes.c_double),
         ("trace_len", ctypes.c_longlong),
+        ("field_io", ctypes.c_longlong),
     ]
 
     def as_dict(self):

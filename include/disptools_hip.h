@@ -118,6 +118,8 @@ typedef struct dt_solve_info {
     double final_max_error;  /* masked max |e| of the returned field        */
     double seconds;          /* wall time of the solve on the device        */
     long long trace_len;     /* log lines produced (it, E, max, eta each)   */
+    long long field_io;      /* [3][nz][ny][nx] field volumes read+written by
+                                the evaluated passes (algorithmic traffic)  */
 } dt_solve_info;
 
 /* Jacobian of a device-resident field (device pointers). Returns 0 or -1. */

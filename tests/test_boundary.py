@@ -1,0 +1,109 @@
+"""The drop-in boundary without a GPU: the C-ABI library loads and exports every
+symbol include/disptools_hip.h declares; the extension has the reference's
+method table and error behaviour on the paths that fail before any device work."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+
+def test_library_exports_every_declared_symbol():
+    from disptools_amd import native
+    names = native.declared_symbols()
+    for required in ("set_device", "generate_displacement_gradient_cuda",
+                     "generate_displacement_greedy_cuda", "jacobian_dynamic", "regularise",
+                     "disptools_get_error", "disptools_has_error", "disptools_clear_error",
+                     "get_float_type_size", "dt_solve_device", "dt_solve_host", "dt_jacobian_device"):
+        assert required in names
+    for n in names:
+        assert hasattr(native.lib, n), n
+
+
+def test_float_size_and_error_api():
+    from disptools_amd import native
+    assert native.lib.get_float_type_size() == 32
+    native.lib.disptools_clear_error()
+    assert not native.lib.disptools_has_error()
+
+
+def test_invalid_arguments_set_the_error_state():
+    from disptools_amd import native
+    with pytest.raises(native.HipError, match="at least 3"):
+        native.solve_host("gradient", np.ones((2, 5, 5), np.float32), np.ones((2, 5, 5), bool))
+    with pytest.raises(native.HipError, match="spacing"):
+        native.solve_host("gradient", np.ones((5, 5, 5), np.float32), np.ones((5, 5, 5), bool),
+                          spacing=(1.0, 0.0, 1.0))
+    info = native.SolveInfo()
+    p = native.make_params()
+    rc = native.lib.dt_solve_host(7, 5, 5, 5, 1.0, 1.0, 1.0, None, None, ctypes.byref(p), None,
+                                  ctypes.byref(info), None, 0)
+    assert rc == -1 and "algorithm" in native.last_error()
+
+
+def test_extension_method_table():
+    from disptools_amd import load_extension
+    ext = load_extension()
+    for m in ("get_float_type_size", "regularise", "jacobian", "displacement", "error_out"):
+        assert hasattr(ext, m)
+    assert ext.get_float_type_size() == 32
+
+
+def _disp_args(**over):
+    J = np.ones((6, 6, 6), np.float32)
+    args = dict(spacing=(1.0, 1.0, 1.0), J=J, mask=J > 0, eps=9.99e-4, tol=0.2, eta=0.1, eta_max=0.4,
+                alpha=1.2, beta=0.5, gamma=0.1, delta=1e-3, zeta=10.0, theta=1e-6, iota=1e-9,
+                strict=False, it_max=10, field=np.zeros((3, 6, 6, 6), np.float32),
+                algorithm="gradient", gpu_id=0)
+    args.update(over)
+    return list(args.values())
+
+
+def test_extension_error_behaviour():
+    from disptools_amd import load_extension
+    ext = load_extension()
+    with pytest.raises(ValueError, match="Invalid algorithm"):
+        ext.displacement(*_disp_args(algorithm="fancy"))
+    with pytest.raises(ValueError, match="gpu_id"):
+        ext.displacement(*_disp_args(gpu_id=-1))
+    with pytest.raises(ValueError, match="dtype"):
+        ext.displacement(*_disp_args(J=np.ones((6, 6, 6), np.float64)))
+    with pytest.raises(ValueError, match="mask"):
+        ext.displacement(*_disp_args(mask=np.ones((6, 6, 5), bool)))
+    with pytest.raises(ValueError, match="spacing"):
+        ext.displacement(*_disp_args(spacing=(1.0, 1.0)))
+    with pytest.raises(ValueError, match="field"):
+        ext.displacement(*_disp_args(field=np.zeros((3, 6, 6, 5), np.float32)))
+
+
+def test_extension_rejects_matching_on_device():
+    from disptools_amd import load_extension, native
+    ext = load_extension()
+    if not native.set_device(0):
+        pytest.skip("no HIP device: the device check fails first")
+    with pytest.raises(ValueError, match="not supported on GPU"):
+        ext.displacement(*_disp_args(algorithm="matching"))
+
+
+def test_levels_above_one_not_reproduced():
+    from disptools_amd import displacements
+    with pytest.raises(NotImplementedError):
+        displacements.displacement(np.ones((8, 8, 8), np.float32), levels=2, gpu_id=0)
+
+
+def test_redistribute_volume_change_closed_form():
+    # restates test/test_displacements.py:57-71
+    from disptools_amd import displacements
+    rng = np.random.default_rng(3)
+    data = rng.random((40, 40, 40)) * 2.0
+    index = rng.random((40, 40, 40)) < 0.2
+    a = displacements.redistribute_volume_change(data, index.astype(np.float32))
+    assert np.array_equal(a[index], data[index])
+    assert abs(np.sum(a) - a.size) < 1e-5
+
+
+def test_sphere_config_geometry():
+    from disptools_amd.workloads import sphere_case
+    T, M, sp = sphere_case(40)
+    assert M.sum() == 4169
+    assert set(np.unique(T)) == {np.float32(1.0), np.float32(1.1)}
