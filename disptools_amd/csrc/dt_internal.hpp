@@ -71,6 +71,7 @@ struct Ctrl {
 struct Inst {
     int nx, ny, nz;
     int ntx, nty, ntz, tz;   // tiles per axis and z-chunk depth
+    int variant;             // column-tile shape of the gradient pass
     long long n;             // voxels
     float ix, iy, iz;        // inverse spacing, (float)(1.0 / d)
     float dx, dy, dz;        // spacing (the greedy step uses it)
@@ -84,7 +85,6 @@ struct Inst {
     Ctrl *ctrl;
     double *trace;           // [trace_cap][4]
     long long trace_cap;
-    int tile_base;           // index of this instance's first partial slot
 };
 
 // One entry of a launch's tile table.

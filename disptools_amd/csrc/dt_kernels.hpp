@@ -14,8 +14,24 @@ constexpr int TILE_Y = 8;
 hipError_t launch_jacobian(const float *f, float *J, int nx, int ny, int nz, float ix, float iy, float iz,
                            hipStream_t st);
 hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st);
+// Variants of the fused gradient pass: column tile TX x TY (one thread per
+// column) and the waves per SIMD its registers are capped for (3: may spill a
+// few registers; 2: no spill, lower occupancy; 4: spills more).
+constexpr int GRAD_VARIANTS = 8;
+inline void grad_variant_dims(int v, int &tx, int &ty)
+{
+    switch (v) {
+    case 1: case 5: case 6: tx = 32; ty = 16; break;
+    case 2: tx = 64; ty = 4; break;
+    case 3: tx = 64; ty = 8; break;
+    default: tx = 32; ty = 8; break;
+    }
+}
 // precision: 32 = FP32 stencil arithmetic, 64 = FP64 stencil arithmetic (reference-exact)
-hipError_t launch_grad_pass(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
+hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
+                            hipStream_t st);
+// resident blocks per CU of a variant (from the compiled resource usage)
+int grad_pass_blocks_per_cu(int precision, int variant);
 hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 

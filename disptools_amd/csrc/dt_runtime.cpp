@@ -144,14 +144,42 @@ static size_t scratch_bytes(int algorithm, long long n, int ntiles, long long tr
     return align_up(b, 256) + 256;
 }
 
-// Default z-chunk depth of a tile.
-static int default_tz(int nx, int ny, int nz)
+// Resident blocks of one launch on the current device (CUs x blocks per CU).
+static int resident_blocks(int precision, int algorithm, int variant)
 {
-    const long long cols = (long long)((nx + TILE_X - 1) / TILE_X) * ((ny + TILE_Y - 1) / TILE_Y);
-    // aim for >= ~1024 blocks while keeping chunks at least 8 planes deep
-    int tz = 32;
-    while (tz > 8 && cols * ((nz + tz - 1) / tz) < 1024) tz /= 2;
-    return std::min(tz, std::max(nz, 1));
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    const int per_cu = algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(precision, variant) : 4;
+    return cus * per_cu;
+}
+
+// z-chunk depth of a tile: a block streams TZ planes after a 2-plane (stage A)
+// and 6-plane (loads) prologue, so its time is ~ TZ + 3 plane steps; pick the
+// depth that minimises (rounds of resident blocks) x (TZ + 3).
+static int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
+{
+    const long long cols = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
+    int best = std::max(1, std::min(nz, 16));
+    double best_cost = 1e300;
+    for (int tz = 4; tz <= std::max(4, nz); ++tz) {
+        const long long blocks = cols * ((nz + tz - 1) / tz);
+        const long long rounds = (blocks + resident - 1) / resident;
+        const double cost = (double)rounds * (std::min(tz, nz) + 3);
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = std::min(tz, nz); }
+    }
+    return best;
+}
+
+static int variant_from_env()
+{
+    const char *s = std::getenv("DISPTOOLS_GRAD_VARIANT");
+    if (!s) return 0;
+    const int v = std::atoi(s);
+    return (v >= 0 && v < GRAD_VARIANTS) ? v : 0;
 }
 
 // Tile table ordered so that block b (dispatched to XCD b % 8) walks a
@@ -181,6 +209,9 @@ static int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, fl
     if (nx < 3 || ny < 3 || nz < 3)
         DT_FAIL("every image dimension must be at least 3 voxels (the stencils read 3 samples)");
     if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) DT_FAIL("image too large");
+    // a field buffer is addressed by one 32-bit byte offset (12 bytes per voxel)
+    if ((unsigned long long)nx * ny * nz * 12ull >= (1ull << 32))
+        DT_FAIL("image too large: at most 357913941 voxels per solve");
     if (!(dx > 0.0f) || !(dy > 0.0f) || !(dz > 0.0f)) DT_FAIL("spacing must be positive");
     if (!p) DT_FAIL("null parameters");
     return 0;
@@ -193,8 +224,12 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
 {
     const auto t_start = std::chrono::steady_clock::now();
     const long long n = (long long)nx * ny * nz;
-    const int tz = default_tz(nx, ny, nz);
-    const int ntx = (nx + TILE_X - 1) / TILE_X, nty = (ny + TILE_Y - 1) / TILE_Y, ntz = (nz + tz - 1) / tz;
+    const int precision = precision_from_env();
+    const int variant = algorithm == DT_GRADIENT ? variant_from_env() : 0;
+    int tx = TILE_X, ty = TILE_Y;
+    if (algorithm == DT_GRADIENT) grad_variant_dims(variant, tx, ty);
+    const int tz = choose_tz(nx, ny, nz, tx, ty, resident_blocks(precision, algorithm, variant));
+    const int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty, ntz = (nz + tz - 1) / tz;
     const int ntiles = ntx * nty * ntz;
     if (trace_cap < 0) trace_cap = 0;
 
@@ -205,6 +240,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     Inst h{};
     h.nx = nx; h.ny = ny; h.nz = nz;
     h.ntx = ntx; h.nty = nty; h.ntz = ntz; h.tz = tz;
+    h.variant = variant;
     h.n = n;
     h.ix = (float)(1.0 / (double)dx);
     h.iy = (float)(1.0 / (double)dy);
@@ -265,7 +301,6 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         DT_HIP(hipMemsetAsync(h.buf[1], 0, 3 * (size_t)n * sizeof(float), st));
     }
 
-    const int precision = precision_from_env();
     hipEvent_t ev[2];
     DT_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     DT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
@@ -278,7 +313,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     auto enqueue = [&](int k, int slot) -> int {
         for (int i = 0; i < k; ++i) {
             if (algorithm == DT_GRADIENT) {
-                DT_HIP(launch_grad_pass(precision, d_inst, d_tiles, ntiles, st));
+                DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, st));
             } else {
                 DT_HIP(launch_greedy_step(d_inst, d_tiles, ntiles, st));
                 DT_HIP(launch_greedy_eval(precision, d_inst, d_tiles, ntiles, st));
@@ -290,7 +325,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         return 0;
     };
 
-    int chunk = 4;
+    // small chunks bound the passes that run idle after the solve has stopped
+    int chunk = 8;
     rc = enqueue(chunk, 0);
     if (!rc) rc = enqueue(chunk, 1);
     int slot = 0;
@@ -303,7 +339,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
             rc = -1;
             break;
         }
-        chunk = std::min(chunk * 2, 64);
+        chunk = std::min(chunk * 2, 16);
         rc = enqueue(chunk, slot);
         slot ^= 1;
     }

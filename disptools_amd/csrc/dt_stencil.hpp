@@ -34,6 +34,17 @@ __device__ __forceinline__ void classify(int x, int y, int z, int nx, int ny, in
     }
 }
 
+// Weights of f(-2) .. f(+2) of each stencil kind (times 1/spacing).
+__device__ __forceinline__ void stencil_weights(int kind, float w[5])
+{
+    const bool c4 = kind == K_C4, c2 = kind == K_C2, fw = kind == K_FWD;
+    w[0] = c4 ? (float)(1.0 / 12.0) : (c2 || fw ? 0.0f : 0.5f);
+    w[1] = c4 ? (float)(-2.0 / 3.0) : (c2 ? -0.5f : (fw ? 0.0f : -2.0f));
+    w[2] = c4 || c2 ? 0.0f : (fw ? -1.5f : 1.5f);
+    w[3] = c4 ? (float)(2.0 / 3.0) : (c2 ? 0.5f : (fw ? 2.0f : 0.0f));
+    w[4] = c4 ? (float)(-1.0 / 12.0) : (c2 ? 0.0f : (fw ? -0.5f : 0.0f));
+}
+
 // Derivative along one axis from the five samples f(-2) .. f(+2).
 template <typename CT>
 __device__ __forceinline__ CT dstencil(int kind, float m2, float m1, float c0, float p1, float p2, float inv)
@@ -66,6 +77,31 @@ __device__ __forceinline__ CT det_plus_identity(const CT d[3][3])
     return a11 * (a22 * a33 - a23 * a32)
          - a12 * (a21 * a33 - a23 * a31)
          + a13 * (a21 * a32 - a22 * a31);
+}
+
+// det(I + D) - 1 = tr D + (sum of the principal 2x2 minors of D) + det D,
+// summed from the smallest terms up.  For |D| << 1 this keeps J - 1 to a
+// relative accuracy of the compute type instead of an absolute error of half
+// an ulp of 1.0.
+template <typename CT>
+__device__ __forceinline__ CT det_plus_identity_minus_one(const CT d[3][3])
+{
+    const CT m01 = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+    const CT m02 = d[0][0] * d[2][2] - d[0][2] * d[2][0];
+    const CT m12 = d[1][1] * d[2][2] - d[1][2] * d[2][1];
+    const CT det = d[0][0] * m12 - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+                   d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+    return ((det + (m01 + m02 + m12)) + (d[0][0] + d[1][1] + d[2][2]));
+}
+
+// Voxel error from dJ = J - 1 and dT = T - 1 (exact in float for T in [0.5, 2]):
+// err = J - T = dJ - dT, then the same branches as voxel_error.
+__device__ __forceinline__ float voxel_error_delta(float dJ, float dT, bool in_mask, float tol)
+{
+    const float err = dJ - dT;
+    if (in_mask) return err;
+    const float a = fabsf(err);
+    return a < tol ? 0.0f : copysignf(a - tol, err);
 }
 
 // Voxel error (src/headers/error.h:128-142): full error inside the mask,
