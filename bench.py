@@ -16,7 +16,11 @@ those bytes / (HIP-event time of the solves on their stream / pass launches).
 
 cpu_baseline: the reference C library built from its sources (oracle/_ref,
 kind "reference"; the oracle restatement, kind "port", if that build is
-absent) on the same 128^3 inputs for a bounded number of iterations.
+absent) solving the same 128^3 inputs on the host cores, repeated for about
+10 s of CPU work.
+
+--mode copy runs a plain device-to-device copy of 1 GiB five times and prints
+its byte count: the calibration of the FETCH_SIZE / WRITE_SIZE counters.
 """
 
 import argparse
@@ -38,32 +42,53 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--it-max", type=int, default=2000)
-    ap.add_argument("--cpu-iters", type=int, default=200,
-                    help="iterations of the CPU baseline sample (0 to skip)")
+    ap.add_argument("--cpu-iters", type=int, default=2000,
+                    help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", choices=("solve", "copy"), default="solve")
     return ap.parse_args()
 
 
-def cpu_baseline(T, M, sp, iters, threads):
+def copy_calibration():
+    """Five copies of a 1 GiB float32 buffer (known bytes read and written)."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "copies": 5, "bytes_read_per_copy": 4 * n,
+                      "bytes_written_per_copy": 4 * n}))
+
+
+def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50):
+    """Solves of the same inputs on the host, repeated until about
+    `min_seconds` of CPU work (bounded by `max_repeats`)."""
     from oracle.oracle import Oracle, Reference, reference_available
     orc = Oracle(32)
     orc.set_threads(threads)   # shared OpenMP runtime: applies to the reference build too
     n = T.size
-    if reference_available(32):
-        ref = Reference(32)
+    done = 0
+    dt = 0.0
+    reps = 0
+    kind = "reference" if reference_available(32) else "port"
+    ref = Reference(32) if kind == "reference" else None
+    while reps < max_repeats and (reps == 0 or dt < min_seconds):
         t0 = time.perf_counter()
-        _, info = ref.solve("gradient", T, M, sp, it_max=iters, capture=True)
-        dt = time.perf_counter() - t0
-        done = max(len(info["lines"]) - 1, 0)
-        kind = "reference"
-    else:
-        t0 = time.perf_counter()
-        _, info = orc.solve("gradient", T, M, sp, it_max=iters)
-        dt = time.perf_counter() - t0
-        done = max(len(info["trace"]) - 1, 0)
-        kind = "port"
+        if ref is not None:
+            _, info = ref.solve("gradient", T, M, sp, it_max=iters, capture=True)
+            it = max(len(info["lines"]) - 1, 0)
+        else:
+            _, info = orc.solve("gradient", T, M, sp, it_max=iters)
+            it = max(len(info["trace"]) - 1, 0)
+        dt += time.perf_counter() - t0
+        done += it
+        reps += 1
     return {"value": n * done / dt, "unit": "voxel-iters/s", "cores": threads, "kind": kind,
-            "sample": "gradient solve of the same 128^3 inputs, first %d iterations (%.1f s)" % (done, dt)}
+            "sample": "%d gradient solve(s) of the same 128^3 inputs (it_max=%d), %d iterations "
+                      "in %.1f s" % (reps, iters, done, dt)}
 
 
 def main():
@@ -72,6 +97,9 @@ def main():
         print("bench.py runs on one GPU only (each GPU box has one device); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.mode == "copy":
+        copy_calibration()
+        return
 
     import numpy as np
     import torch

@@ -37,6 +37,7 @@ HIP_FLAGS = [
 # bit-exact with the reference); the solver pass is FP32 with FMA contraction.
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_grad_pass.hip", "-ffp-contract=fast"),
+               ("dt_split_pass.hip", "-ffp-contract=fast"),
                ("dt_runtime.cpp", "-ffp-contract=off")]
 
 

@@ -4,6 +4,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #include "dt_internal.hpp"
 #include "dt_stencil.hpp"
 

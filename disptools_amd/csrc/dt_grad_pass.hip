@@ -27,6 +27,7 @@
 // file is the solver's; the bit-exact standalone jacobian is separate).
 #include "dt_device.hpp"
 #include "dt_kernels.hpp"
+#include "dt_pass_common.hpp"
 
 // Planes of trial-field prefetch in registers (1 or 2).
 #ifndef DT_PREFETCH_DIST
@@ -47,32 +48,6 @@ struct GradSmem {
 
 __device__ __forceinline__ int ring5(int z) { return (z + 10) % 5; }
 
-constexpr int RSRC_WORD3 = 0x00020000;   // raw buffer, gfx9-family data format
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
-}
-
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
-}
-
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
-}
-
-struct PassParams {
-    int nx, ny, nz;
-    float inv[3];     // 1 / spacing
-    float c4a[3];     // (2/3) * inv
-    float c4b[3];     // (1/12) * inv
-    float c2[3];      // 0.5 * inv
-    float tol, delta, zeta;
-};
-
 // Stage A at one point: samples s[c][axis][m2, m1, p1, p2] and centre v0[c].
 // what: 0 = tile point (all W), 1 = x-ring (W_{a,x} only), 2 = y-ring (W_{a,y} only).
 template <typename CT, int TX, int TY>
@@ -80,42 +55,15 @@ __device__ __forceinline__ void stage_a(GradSmem<TX, TY> &S, const PassParams &P
                                         const float (&sm)[3][3][4], const float (&v0)[3], int wi, float T,
                                         bool in_mask, int what, float wz[3], bool count, double &Esum, float &mx)
 {
-    CT d4[3][3];
-    const bool fast = gx >= 2 && gx <= P.nx - 3 && gy >= 2 && gy <= P.ny - 3 && zz >= 2 && zz <= P.nz - 3;
-    // wave-uniform choice: waves with no boundary lane take the 4th-order form,
-    // the others a weighted 5-point form whose weights encode each lane's
-    // stencil (no divergent branches on the region map)
-    if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-                d4[c][a] = (CT)(sm[c][a][2] - sm[c][a][1]) * (CT)P.c4a[a] -
-                           (CT)(sm[c][a][3] - sm[c][a][0]) * (CT)P.c4b[a];
-    } else {
-        int kind[3];
-        classify(gx, gy, zz, P.nx, P.ny, P.nz, kind);
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float w[5];
-            stencil_weights(kind[a], w);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                d4[c][a] = ((CT)w[0] * sm[c][a][0] + (CT)w[1] * sm[c][a][1] + (CT)w[2] * v0[c] +
-                            (CT)w[3] * sm[c][a][2] + (CT)w[4] * sm[c][a][3]) * (CT)P.inv[a];
-        }
-    }
     // J - 1 and J - T without the rounding of J near 1.0
-    const float dJ = (float)det_plus_identity_minus_one<CT>(d4);
+    const float dJ = jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     if (count) {
         Esum += (double)e * (double)e;
         if (in_mask) mx = fmaxf(mx, fabsf(e));
     }
-    float w = e;
-    if (J < P.delta) w += P.zeta * (J - P.delta);
-    const CT cw = (CT)w;
+    const CT cw = (CT)penalised_weight(e, J, P);
 
     // 2nd-order central derivatives (gradient.c:13-21), identity on the diagonal
     CT d2[3][3];
@@ -163,16 +111,7 @@ __device__ __forceinline__ void grad_pass_body(GradSmem<TX, TY> &S, const Inst &
     constexpr int NHALO = 2 * TX + 2 * TY;
     constexpr int PER = (3 * PL + NT - 1) / NT;
 
-    PassParams P;
-    P.nx = uni(I.nx); P.ny = uni(I.ny); P.nz = uni(I.nz);
-    P.inv[0] = uni(I.ix); P.inv[1] = uni(I.iy); P.inv[2] = uni(I.iz);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        P.c4a[a] = (float)(2.0 / 3.0) * P.inv[a];
-        P.c4b[a] = (float)(1.0 / 12.0) * P.inv[a];
-        P.c2[a] = 0.5f * P.inv[a];
-    }
-    P.tol = uni(I.p.tolerance); P.delta = uni(I.p.delta); P.zeta = uni(I.p.zeta);
+    const PassParams P = load_params(I);
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const unsigned n = uni((unsigned)I.n);
     const unsigned plane = (unsigned)nx * (unsigned)ny;
@@ -496,12 +435,14 @@ static int occupancy_ct(int variant)
 hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                             hipStream_t st)
 {
+    if (variant >= SPLIT_BASE) return launch_split_pass(precision, variant, insts, tiles, nblocks, st);
     return precision == 64 ? launch_ct<double>(variant, insts, tiles, nblocks, st)
                            : launch_ct<float>(variant, insts, tiles, nblocks, st);
 }
 
 int grad_pass_blocks_per_cu(int precision, int variant)
 {
+    if (variant >= SPLIT_BASE) return split_pass_blocks_per_cu(precision, variant);
     return precision == 64 ? occupancy_ct<double>(variant) : occupancy_ct<float>(variant);
 }
 

@@ -80,6 +80,7 @@ struct Inst {
     const uint8_t *mask;
     float *buf[NBUF];
     float *err[2];           // greedy error maps
+    float *wmap;             // penalised weight per voxel (two-kernel gradient pass)
     double *partials;        // [tiles][4]: sum e^2, max |e| (masked), sum |g|^2, spare
     unsigned int *counter;   // blocks of this instance that finished the pass
     Ctrl *ctrl;

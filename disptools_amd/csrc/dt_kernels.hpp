@@ -17,10 +17,15 @@ hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st);
 // Variants of the fused gradient pass: column tile TX x TY (one thread per
 // column) and the waves per SIMD its registers are capped for (3: may spill a
 // few registers; 2: no spill, lower occupancy; 4: spills more).
-constexpr int GRAD_VARIANTS = 8;
+// Variants from SPLIT_BASE on run each pass as two kernels (dt_split_pass.hip).
+constexpr int GRAD_VARIANTS = 11;
+constexpr int SPLIT_BASE = 8;
 inline void grad_variant_dims(int v, int &tx, int &ty)
 {
     switch (v) {
+    case SPLIT_BASE: tx = 32; ty = 16; break;
+    case SPLIT_BASE + 1: tx = 32; ty = 8; break;
+    case SPLIT_BASE + 2: tx = 64; ty = 8; break;
     case 1: case 5: case 6: tx = 32; ty = 16; break;
     case 2: tx = 64; ty = 4; break;
     case 3: tx = 64; ty = 8; break;
@@ -32,6 +37,9 @@ hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const
                             hipStream_t st);
 // resident blocks per CU of a variant (from the compiled resource usage)
 int grad_pass_blocks_per_cu(int precision, int variant);
+hipError_t launch_split_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
+                             hipStream_t st);
+int split_pass_blocks_per_cu(int precision, int variant);
 hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 

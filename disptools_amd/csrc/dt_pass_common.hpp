@@ -1,0 +1,90 @@
+// dt_pass_common.hpp -- buffer addressing and per-launch constants shared by
+// the solver pass kernels.
+#pragma once
+
+#include "dt_device.hpp"
+
+namespace dt {
+
+constexpr int RSRC_WORD3 = 0x00020000;   // raw buffer, gfx9-family data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, 0);
+}
+
+struct PassParams {
+    int nx, ny, nz;
+    float inv[3];     // 1 / spacing
+    float c4a[3];     // (2/3) * inv
+    float c4b[3];     // (1/12) * inv
+    float c2[3];      // 0.5 * inv
+    float tol, delta, zeta;
+};
+
+__device__ __forceinline__ PassParams load_params(const Inst &I)
+{
+    PassParams P;
+    P.nx = uni(I.nx); P.ny = uni(I.ny); P.nz = uni(I.nz);
+    P.inv[0] = uni(I.ix); P.inv[1] = uni(I.iy); P.inv[2] = uni(I.iz);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        P.c4a[a] = (float)(2.0 / 3.0) * P.inv[a];
+        P.c4b[a] = (float)(1.0 / 12.0) * P.inv[a];
+        P.c2[a] = 0.5f * P.inv[a];
+    }
+    P.tol = uni(I.p.tolerance); P.delta = uni(I.p.delta); P.zeta = uni(I.p.zeta);
+    return P;
+}
+
+// J - 1 at voxel (gx, gy, zz) from its samples sm[c][axis][m2, m1, p1, p2] and
+// centre v0[c], with the stencil the region map assigns there.  Waves with no
+// boundary lane take the 4th-order form; the others a weighted 5-point form
+// whose weights encode each lane's stencil (no divergent branches).
+template <typename CT>
+__device__ __forceinline__ float jacobian_delta(const PassParams &P, int gx, int gy, int zz,
+                                                const float (&sm)[3][3][4], const float (&v0)[3])
+{
+    CT d4[3][3];
+    const bool fast = gx >= 2 && gx <= P.nx - 3 && gy >= 2 && gy <= P.ny - 3 && zz >= 2 && zz <= P.nz - 3;
+    if (__builtin_amdgcn_ballot_w64(!fast) == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+                d4[c][a] = (CT)(sm[c][a][2] - sm[c][a][1]) * (CT)P.c4a[a] -
+                           (CT)(sm[c][a][3] - sm[c][a][0]) * (CT)P.c4b[a];
+    } else {
+        int kind[3];
+        classify(gx, gy, zz, P.nx, P.ny, P.nz, kind);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float w[5];
+            stencil_weights(kind[a], w);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                d4[c][a] = ((CT)w[0] * sm[c][a][0] + (CT)w[1] * sm[c][a][1] + (CT)w[2] * v0[c] +
+                            (CT)w[3] * sm[c][a][2] + (CT)w[4] * sm[c][a][3]) * (CT)P.inv[a];
+        }
+    }
+    return (float)det_plus_identity_minus_one<CT>(d4);
+}
+
+// Penalised weight of the adjoint stencil (displacement_field_gradient.c:71-88):
+// w = e + zeta (J - delta) where J < delta.
+__device__ __forceinline__ float penalised_weight(float e, float J, const PassParams &P)
+{
+    return J < P.delta ? e + P.zeta * (J - P.delta) : e;
+}
+
+}  // namespace dt

@@ -135,6 +135,7 @@ static size_t scratch_bytes(int algorithm, long long n, int ntiles, long long tr
     const int extra_fields = algorithm == DT_GRADIENT ? NBUF - 1 : 1;
     for (int i = 0; i < extra_fields; ++i) add(3 * (size_t)n * sizeof(float));
     if (algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
+    if (algorithm == DT_GRADIENT) add((size_t)n * sizeof(float));
     add((size_t)ntiles * 4 * sizeof(double));
     add(sizeof(unsigned int));
     add(sizeof(Ctrl));
@@ -174,12 +175,16 @@ static int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
     return best;
 }
 
+// Default pass variant: the fused pass on 32 x 16 column tiles (measured fastest
+// at 128^3 among the compiled variants); DISPTOOLS_GRAD_VARIANT overrides it.
+constexpr int DEFAULT_GRAD_VARIANT = 1;
+
 static int variant_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_GRAD_VARIANT");
-    if (!s) return 0;
+    if (!s) return DEFAULT_GRAD_VARIANT;
     const int v = std::atoi(s);
-    return (v >= 0 && v < GRAD_VARIANTS) ? v : 0;
+    return (v >= 0 && v < GRAD_VARIANTS) ? v : DEFAULT_GRAD_VARIANT;
 }
 
 // Tile table ordered so that block b (dispatched to XCD b % 8) walks a
@@ -257,6 +262,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         h.err[0] = cv.take<float>((size_t)n);
         h.err[1] = cv.take<float>((size_t)n);
     }
+    h.wmap = algorithm == DT_GRADIENT ? cv.take<float>((size_t)n) : nullptr;
     h.partials = cv.take<double>((size_t)ntiles * 4);
     h.counter = cv.take<unsigned int>(1);
     h.ctrl = cv.take<Ctrl>(1);
@@ -326,7 +332,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     };
 
     // small chunks bound the passes that run idle after the solve has stopped
-    int chunk = 8;
+    // (at most two chunks); two chunks in flight hide the host turnaround
+    int chunk = 4;
     rc = enqueue(chunk, 0);
     if (!rc) rc = enqueue(chunk, 1);
     int slot = 0;
@@ -339,7 +346,6 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
             rc = -1;
             break;
         }
-        chunk = std::min(chunk * 2, 16);
         rc = enqueue(chunk, slot);
         slot ^= 1;
     }
