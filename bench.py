@@ -34,6 +34,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel-iters/s at 128³ + p50/p99 solve latency @ fixed QPS vs plain GPU solve"
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_grad_pass launch of the default solve (128^3, it_max 2000,
+# default pass variant), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
+# runs of this script, scaled by what the counters read for a 1 GiB copy
+# (--mode copy): read bytes = 2.0 x FETCH_SIZE, written = 1.0 x WRITE_SIZE.
+MEASURED_TRAFFIC_128 = 68.85e6
 
 
 def parse():
@@ -155,6 +160,7 @@ def main():
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     cpu = cpu_baseline(T, M, sp, args.cpu_iters, threads) if args.cpu_iters > 0 else None
 
+    default_workload = args.it_max == 2000 and "DISPTOOLS_GRAD_VARIANT" not in os.environ
     last = infos[-1]
     out = {
         "metric": METRIC,
@@ -173,7 +179,8 @@ def main():
                                "target, ellipsoid mask, zero init, it_max=%d" % args.it_max,
                    "voxels": n, "it_max": args.it_max},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": MEASURED_TRAFFIC_128 if default_workload else None,
                      "kernel": "k_grad_pass", "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes / launches},
         "cpu_baseline": cpu,

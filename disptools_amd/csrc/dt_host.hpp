@@ -1,0 +1,105 @@
+// dt_host.hpp -- host-side pieces shared by the single-volume solve and the
+// batch engine: error reporting, instance planning (tile shape, z-chunk depth,
+// device buffers), the initial control block and the XCD-aware tile table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <vector>
+
+#include "disptools_hip.h"
+#include "dt_internal.hpp"
+#include "dt_kernels.hpp"
+
+namespace dt {
+
+// Sets the process-global error state (the contract of src/disptools.c:8-43).
+void set_error(int line, const char *file, const char *func, const char *msg);
+
+#define DT_FAIL(msg)                                         \
+    do {                                                     \
+        ::dt::set_error(__LINE__, __FILE__, __func__, (msg)); \
+        return -1;                                           \
+    } while (0)
+
+#define DT_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            char m_[1024];                                                            \
+            std::snprintf(m_, sizeof m_, "%s: %s", #call, hipGetErrorString(e_));     \
+            ::dt::set_error(__LINE__, __FILE__, __func__, m_);                        \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Carves 256-byte aligned sub-buffers out of a base pointer.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    explicit Carver(void *b) : base((char *)b) {}
+    template <typename T>
+    T *take(size_t count)
+    {
+        off = align_up(off, 256);
+        T *p = (T *)(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+// Tiling of one instance.
+struct Plan {
+    int algorithm = DT_GRADIENT;
+    int precision = 32;      // stencil arithmetic: 32 or 64 bits
+    int variant = 0;         // gradient pass variant (column tile shape)
+    int nx = 0, ny = 0, nz = 0;
+    int tx = TILE_X, ty = TILE_Y, tz = 1;
+    int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
+    long long n = 0;
+};
+
+// Blocks of one pass launch resident at once on the current device.
+int resident_blocks(int precision, int algorithm, int variant);
+// z-chunk depth that minimises (rounds of resident blocks) x (TZ + 3).
+int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident);
+// DISPTOOLS_GRAD_VARIANT / DISPTOOLS_STENCIL_PRECISION overrides.
+int variant_from_env();
+int precision_from_env();
+// Tiling of an instance; tz <= 0 picks the depth for a launch of this
+// instance alone (choose_tz).
+Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant, int tz);
+
+// Device bytes of an instance's solver state besides target, mask and the
+// caller's field (extra field buffers, error maps, tile partials, trace; the
+// control block and counter too when `own_ctrl`).
+size_t scratch_bytes(const Plan &P, long long trace_cap, bool own_ctrl);
+
+// Fills the instance record: geometry, parameters, the caller's buffers and
+// the solver state carved from `cv` (same order and sizes as scratch_bytes).
+// ctrl / counter are carved too when null.
+Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, const float *target,
+               const uint8_t *mask, float *field, Carver &cv, double *trace, long long trace_cap, Ctrl *ctrl,
+               unsigned int *counter);
+
+// Control block of a solve that starts from the initial guess in buf[0].
+Ctrl initial_ctrl(int algorithm, const dt_params &p);
+
+// Appends the tiles of `segments` (instance index, tile count) to `out`, in
+// the order that gives block b (dispatched to XCD b % 8) a contiguous run of
+// the concatenated tile list, so the halos a block re-reads sit in its own
+// XCD's L2.
+void append_tiles_xcd(std::vector<Tile> &out, const std::vector<std::pair<int, int>> &segments);
+
+// Checks the arguments of a solve (sets the error state and returns -1).
+int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const dt_params *p);
+
+// Largest number of passes a solve may take before the host declares it
+// stuck (a line search that never terminates, e.g. alpha <= 1 or beta >= 1).
+long long max_passes(const dt_params &p);
+
+}  // namespace dt

@@ -13,11 +13,13 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
 
 #include "disptools_hip.h"
+#include "dt_host.hpp"
 #include "dt_internal.hpp"
 #include "dt_kernels.hpp"
 
@@ -27,7 +29,7 @@
 
 static disptools_error_state g_error = {false, -1, "", "", "", ""};
 
-static void set_error(int line, const char *file, const char *func, const char *msg)
+void dt::set_error(int line, const char *file, const char *func, const char *msg)
 {
     g_error.error = true;
     g_error.line = line;
@@ -36,23 +38,6 @@ static void set_error(int line, const char *file, const char *func, const char *
     std::snprintf(g_error.message, sizeof g_error.message, "%s", msg);
     g_error.trace[0] = '\0';
 }
-
-#define DT_FAIL(msg)                                         \
-    do {                                                     \
-        set_error(__LINE__, __FILE__, __func__, (msg));      \
-        return -1;                                           \
-    } while (0)
-
-#define DT_HIP(call)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (call);                                                       \
-        if (e_ != hipSuccess) {                                                       \
-            char m_[1024];                                                            \
-            std::snprintf(m_, sizeof m_, "%s: %s", #call, hipGetErrorString(e_));     \
-            set_error(__LINE__, __FILE__, __func__, m_);                              \
-            return -1;                                                                \
-        }                                                                             \
-    } while (0)
 
 extern "C" disptools_error_state *disptools_get_error(void) { return &g_error; }
 extern "C" bool disptools_has_error(void) { return g_error.error; }
@@ -87,8 +72,6 @@ struct Arena {
 static std::mutex g_lock;
 static std::vector<Arena> g_arenas;
 
-static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
 static int arena_get(size_t bytes, Arena **out)
 {
     int dev = 0;
@@ -112,41 +95,91 @@ static int arena_get(size_t bytes, Arena **out)
     return 0;
 }
 
-// Carves aligned sub-buffers out of a base pointer.
-struct Carver {
-    char *base;
-    size_t off = 0;
-    explicit Carver(void *b) : base((char *)b) {}
-    template <typename T>
-    T *take(size_t count)
-    {
-        off = align_up(off, 256);
-        T *p = (T *)(base + off);
-        off += count * sizeof(T);
-        return p;
-    }
-};
+// ---------------------------------------------------------------------------
+// Instance planning (shared with the batch engine)
+// ---------------------------------------------------------------------------
 
-// Bytes of device scratch a solve needs besides the caller's buffers.
-static size_t scratch_bytes(int algorithm, long long n, int ntiles, long long trace_cap)
+size_t scratch_bytes(const Plan &P, long long trace_cap, bool own_ctrl)
 {
     size_t b = 0;
     auto add = [&](size_t s) { b = align_up(b, 256) + s; };
-    const int extra_fields = algorithm == DT_GRADIENT ? NBUF - 1 : 1;
+    const long long n = P.n;
+    const int extra_fields = P.algorithm == DT_GRADIENT ? NBUF - 1 : 1;
     for (int i = 0; i < extra_fields; ++i) add(3 * (size_t)n * sizeof(float));
-    if (algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
-    if (algorithm == DT_GRADIENT) add((size_t)n * sizeof(float));
-    add((size_t)ntiles * 4 * sizeof(double));
-    add(sizeof(unsigned int));
-    add(sizeof(Ctrl));
-    add((size_t)std::max<long long>(trace_cap, 1) * 4 * sizeof(double));
-    add(sizeof(Inst));
-    add((size_t)ntiles * sizeof(Tile));
-    return align_up(b, 256) + 256;
+    if (P.algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
+    if (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE) add((size_t)n * sizeof(float));
+    add((size_t)P.ntiles * 4 * sizeof(double));
+    if (trace_cap > 0) add((size_t)trace_cap * 4 * sizeof(double));
+    if (own_ctrl) { add(sizeof(unsigned int)); add(sizeof(Ctrl)); }
+    return align_up(b, 256);
 }
 
-// Resident blocks of one launch on the current device (CUs x blocks per CU).
-static int resident_blocks(int precision, int algorithm, int variant)
+Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, const float *target,
+               const uint8_t *mask, float *field, Carver &cv, double *trace, long long trace_cap, Ctrl *ctrl,
+               unsigned int *counter)
+{
+    Inst h{};
+    h.nx = P.nx; h.ny = P.ny; h.nz = P.nz;
+    h.ntx = P.ntx; h.nty = P.nty; h.ntz = P.ntz; h.tz = P.tz;
+    h.variant = P.variant;
+    h.n = P.n;
+    h.ix = (float)(1.0 / (double)dx);
+    h.iy = (float)(1.0 / (double)dy);
+    h.iz = (float)(1.0 / (double)dz);
+    h.dx = dx; h.dy = dy; h.dz = dz;
+    h.p = p;
+    h.target = target;
+    h.mask = mask;
+    h.buf[0] = field;
+    const long long n = P.n;
+    const int extra_fields = P.algorithm == DT_GRADIENT ? NBUF - 1 : 1;
+    for (int i = 1; i <= extra_fields; ++i) h.buf[i] = cv.take<float>(3 * (size_t)n);
+    for (int i = extra_fields + 1; i < NBUF; ++i) h.buf[i] = nullptr;
+    if (P.algorithm == DT_GREEDY) {
+        h.err[0] = cv.take<float>((size_t)n);
+        h.err[1] = cv.take<float>((size_t)n);
+    }
+    h.wmap = (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE) ? cv.take<float>((size_t)n) : nullptr;
+    h.partials = cv.take<double>((size_t)P.ntiles * 4);
+    if (trace_cap > 0) {
+        double *t = cv.take<double>((size_t)trace_cap * 4);
+        h.trace = trace ? t : nullptr;
+        h.trace_cap = trace ? trace_cap : 0;
+    } else {
+        h.trace = nullptr;
+        h.trace_cap = 0;
+    }
+    h.counter = counter ? counter : cv.take<unsigned int>(1);
+    h.ctrl = ctrl ? ctrl : cv.take<Ctrl>(1);
+    return h;
+}
+
+Ctrl initial_ctrl(int algorithm, const dt_params &p)
+{
+    Ctrl c{};
+    c.active = 1;
+    c.phase = PH_INIT;
+    c.mode = TRIAL_DIRECT;
+    c.src = 0;
+    c.cand = 0;
+    c.base = 0;
+    c.eta = p.eta;
+    c.out_trial = -1;
+    if (algorithm == DT_GRADIENT) {
+        c.out_next = 1;
+        c.eta_next = p.eta * p.alpha;   // first search step (displacement_field_gradient.c:290-291)
+    } else {
+        c.out_next = -1;
+        c.step_on = 0;
+        c.g_new = 1;
+        c.e_cur = 1;
+        c.e_new = 0;
+    }
+    c.result_buf = 0;
+    return c;
+}
+
+int resident_blocks(int precision, int algorithm, int variant)
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -158,10 +191,9 @@ static int resident_blocks(int precision, int algorithm, int variant)
     return cus * per_cu;
 }
 
-// z-chunk depth of a tile: a block streams TZ planes after a 2-plane (stage A)
-// and 6-plane (loads) prologue, so its time is ~ TZ + 3 plane steps; pick the
-// depth that minimises (rounds of resident blocks) x (TZ + 3).
-static int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
+// A block streams TZ planes after a 2-plane (stage A) and 6-plane (loads)
+// prologue, so its time is ~ TZ + 3 plane steps.
+int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
 {
     const long long cols = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
     int best = std::max(1, std::min(nz, 16));
@@ -179,7 +211,7 @@ static int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
 // at 128^3 among the compiled variants); DISPTOOLS_GRAD_VARIANT overrides it.
 constexpr int DEFAULT_GRAD_VARIANT = 1;
 
-static int variant_from_env()
+int variant_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_GRAD_VARIANT");
     if (!s) return DEFAULT_GRAD_VARIANT;
@@ -187,28 +219,54 @@ static int variant_from_env()
     return (v >= 0 && v < GRAD_VARIANTS) ? v : DEFAULT_GRAD_VARIANT;
 }
 
-// Tile table ordered so that block b (dispatched to XCD b % 8) walks a
-// contiguous range of tiles on each XCD, keeping halo re-reads in one L2.
-static void build_tiles(std::vector<Tile> &out, int inst, int ntiles)
-{
-    const int nx = 8;
-    const int per = (ntiles + nx - 1) / nx;
-    for (int k = 0; k < per; ++k)
-        for (int x = 0; x < nx; ++x) {
-            const int t = x * per + k;
-            if (t < ntiles) out.push_back(Tile{inst, t});
-        }
-}
-
-static int precision_from_env()
+int precision_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_STENCIL_PRECISION");
     if (s && std::strcmp(s, "64") == 0) return 64;
     return 32;
 }
 
-static int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
-                    const dt_params *p)
+Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant, int tz)
+{
+    Plan P;
+    P.algorithm = algorithm;
+    P.precision = precision;
+    P.variant = algorithm == DT_GRADIENT ? variant : 0;
+    P.nx = nx; P.ny = ny; P.nz = nz;
+    P.n = (long long)nx * ny * nz;
+    P.tx = TILE_X; P.ty = TILE_Y;
+    if (algorithm == DT_GRADIENT) grad_variant_dims(P.variant, P.tx, P.ty);
+    P.tz = tz > 0 ? std::min(tz, nz)
+                  : choose_tz(nx, ny, nz, P.tx, P.ty, resident_blocks(precision, algorithm, P.variant));
+    P.ntx = (nx + P.tx - 1) / P.tx;
+    P.nty = (ny + P.ty - 1) / P.ty;
+    P.ntz = (nz + P.tz - 1) / P.tz;
+    P.ntiles = P.ntx * P.nty * P.ntz;
+    return P;
+}
+
+void append_tiles_xcd(std::vector<Tile> &out, const std::vector<std::pair<int, int>> &segments)
+{
+    constexpr int XCDS = 8;
+    std::vector<Tile> flat;
+    for (const auto &s : segments)
+        for (int t = 0; t < s.second; ++t) flat.push_back(Tile{s.first, t});
+    const size_t total = flat.size();
+    const size_t per = (total + XCDS - 1) / XCDS;
+    for (size_t k = 0; k < per; ++k)
+        for (size_t x = 0; x < XCDS; ++x) {
+            const size_t t = x * per + k;
+            if (t < total) out.push_back(flat[t]);
+        }
+}
+
+long long max_passes(const dt_params &p)
+{
+    const long long per_iteration = 64;
+    return 4096 + per_iteration * (std::max<long long>(p.it_max, 0) + 2);
+}
+
+int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const dt_params *p)
 {
     if (algorithm != DT_GRADIENT && algorithm != DT_GREEDY) DT_FAIL("invalid algorithm");
     if (nx < 3 || ny < 3 || nz < 3)
@@ -228,74 +286,26 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
                         dt_solve_info *info, double *trace, long long trace_cap, hipStream_t st)
 {
     const auto t_start = std::chrono::steady_clock::now();
-    const long long n = (long long)nx * ny * nz;
     const int precision = precision_from_env();
-    const int variant = algorithm == DT_GRADIENT ? variant_from_env() : 0;
-    int tx = TILE_X, ty = TILE_Y;
-    if (algorithm == DT_GRADIENT) grad_variant_dims(variant, tx, ty);
-    const int tz = choose_tz(nx, ny, nz, tx, ty, resident_blocks(precision, algorithm, variant));
-    const int ntx = (nx + tx - 1) / tx, nty = (ny + ty - 1) / ty, ntz = (nz + tz - 1) / tz;
-    const int ntiles = ntx * nty * ntz;
+    const Plan P = make_plan(algorithm, nx, ny, nz, precision, variant_from_env(), 0);
+    const long long n = P.n;
+    const int variant = P.variant;
+    const int ntiles = P.ntiles;
     if (trace_cap < 0) trace_cap = 0;
+    if (!trace) trace_cap = 0;
 
     Arena *A = nullptr;
-    if (arena_get(scratch_bytes(algorithm, n, ntiles, trace_cap), &A)) return -1;
+    const size_t own = scratch_bytes(P, trace_cap, true);
+    if (arena_get(own + align_up(sizeof(Inst), 256) + (size_t)ntiles * sizeof(Tile) + 256, &A)) return -1;
     Carver cv(A->ptr);
-
-    Inst h{};
-    h.nx = nx; h.ny = ny; h.nz = nz;
-    h.ntx = ntx; h.nty = nty; h.ntz = ntz; h.tz = tz;
-    h.variant = variant;
-    h.n = n;
-    h.ix = (float)(1.0 / (double)dx);
-    h.iy = (float)(1.0 / (double)dy);
-    h.iz = (float)(1.0 / (double)dz);
-    h.dx = dx; h.dy = dy; h.dz = dz;
-    h.p = *p;
-    h.target = d_target;
-    h.mask = d_mask;
-    h.buf[0] = d_field;
-    const int extra_fields = algorithm == DT_GRADIENT ? NBUF - 1 : 1;
-    for (int i = 1; i <= extra_fields; ++i) h.buf[i] = cv.take<float>(3 * (size_t)n);
-    for (int i = extra_fields + 1; i < NBUF; ++i) h.buf[i] = nullptr;
-    if (algorithm == DT_GREEDY) {
-        h.err[0] = cv.take<float>((size_t)n);
-        h.err[1] = cv.take<float>((size_t)n);
-    }
-    h.wmap = algorithm == DT_GRADIENT ? cv.take<float>((size_t)n) : nullptr;
-    h.partials = cv.take<double>((size_t)ntiles * 4);
-    h.counter = cv.take<unsigned int>(1);
-    h.ctrl = cv.take<Ctrl>(1);
-    h.trace = cv.take<double>((size_t)std::max<long long>(trace_cap, 1) * 4);
-    h.trace_cap = trace ? trace_cap : 0;
-    if (!trace) h.trace = nullptr;
+    const Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, nullptr, nullptr);
     Inst *d_inst = cv.take<Inst>(1);
     Tile *d_tiles = cv.take<Tile>((size_t)ntiles);
 
     std::vector<Tile> tiles;
     tiles.reserve(ntiles);
-    build_tiles(tiles, 0, ntiles);
-
-    Ctrl c{};
-    c.active = 1;
-    c.phase = PH_INIT;
-    c.mode = TRIAL_DIRECT;
-    c.src = 0;
-    c.cand = 0;
-    c.base = 0;
-    c.eta = p->eta;
-    c.out_trial = -1;
-    if (algorithm == DT_GRADIENT) {
-        c.out_next = 1;
-        c.eta_next = p->eta * p->alpha;   // first search step (:290-291)
-    } else {
-        c.out_next = -1;
-        c.step_on = 0;
-        c.g_new = 1;
-        c.e_cur = 1;
-        c.e_new = 0;
-    }
-    c.result_buf = 0;
+    append_tiles_xcd(tiles, {{0, ntiles}});
+    const Ctrl c = initial_ctrl(algorithm, *p);
 
     DT_HIP(hipMemcpyAsync(d_inst, &h, sizeof(Inst), hipMemcpyHostToDevice, st));
     DT_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
@@ -312,8 +322,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     DT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
 
     // passes needed in the worst case before the controller stops the solve
-    const long long per_iteration = 64;
-    const long long max_passes = 4096 + per_iteration * (std::max<long long>(p->it_max, 0) + 2);
+    const long long pass_cap = max_passes(*p);
     long long launched = 0;
     int rc = 0;
     auto enqueue = [&](int k, int slot) -> int {
@@ -341,7 +350,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     while (!rc) {
         if (hipEventSynchronize(ev[slot]) != hipSuccess) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
         if (A->host_ctrl[slot].phase == PH_DONE) { done = true; break; }
-        if (launched > max_passes) {
+        if (launched > pass_cap) {
             set_error(__LINE__, __FILE__, __func__, "solver did not terminate (check alpha > 1 and beta < 1)");
             rc = -1;
             break;
