@@ -50,7 +50,13 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=("solve", "copy"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy"), default="solve")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="volumes of the batched-vs-plain comparison (0 to skip it)")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="requests of the serving stream (0 to skip it)")
+    ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--graphs", type=int, default=1, help="engine replays chunks from hipGraphs")
     return ap.parse_args()
 
 
@@ -96,6 +102,215 @@ def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50):
                       "in %.1f s" % (reps, iters, done, dt)}
 
 
+def _iterations(info):
+    """Iterations performed: the counter ends at it_max + 1 after a full run,
+    and at the iteration whose trial triggered the stop otherwise."""
+    return info["iterations"] - (1 if info["reason"] == 6 else 0)
+
+
+def batch_bench(count, it_max, graphs):
+    """64 volumes of uneven sizes (each axis U{48..160}, spacing 1): solved one
+    dt_solve_device call after the other (the plain one-volume-per-launch
+    solve) and all at once by the batch engine, inputs resident in HBM."""
+    import numpy as np
+    import torch
+    from disptools_amd import native
+    from disptools_amd.workloads import batch_uneven
+
+    cases = batch_uneven(count)
+    dev = torch.device("cuda:0")
+    vols = []
+    for T, M, sp in cases:
+        vols.append((torch.from_numpy(T).to(dev), torch.from_numpy(M.astype(np.uint8)).to(dev),
+                     torch.zeros((3,) + T.shape, dtype=torch.float32, device=dev), sp))
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    def plain(i):
+        T, M, F, sp = vols[i]
+        nz, ny, nx = T.shape
+        info, _ = native.solve_device("gradient", nx, ny, nz, sp, T.data_ptr(), M.data_ptr(),
+                                      F.data_ptr(), stream=sh, it_max=it_max)
+        return info
+
+    # warm-up of both paths on the first volume
+    vols[0][2].zero_()
+    plain(0)
+    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+        vols[0][2].zero_()
+        T, M, F, sp = vols[0]
+        eng.submit("gradient", T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
+                   F.data_ptr(), it_max=it_max)
+        eng.drain()
+        eng.poll()
+    torch.cuda.synchronize()
+
+    for v in vols:
+        v[2].zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_infos = [plain(i) for i in range(len(vols))]
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain_max = [i["final_max_error"] for i in plain_infos]
+
+    for v in vols:
+        v[2].zero_()
+    torch.cuda.synchronize()
+    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+        t0 = time.perf_counter()
+        tickets = {}
+        for i, (T, M, F, sp) in enumerate(vols):
+            nz, ny, nx = T.shape
+            t = eng.submit("gradient", nx, ny, nz, sp, T.data_ptr(), M.data_ptr(), F.data_ptr(),
+                           it_max=it_max)
+            if t is None:
+                raise RuntimeError("the engine's pool cannot hold the batch")
+            tickets[t] = i
+        eng.drain()
+        res = {tickets[t]: (st, info) for t, st, info in eng.poll()}
+        torch.cuda.synchronize()
+        batch_wall = time.perf_counter() - t0
+        stats = eng.stats()
+    batch_infos = [res[i][1] for i in range(len(vols))]
+
+    voxels = [int(np.prod(v[0].shape)) for v in vols]
+    plain_vi = sum(n * _iterations(i) for n, i in zip(voxels, plain_infos))
+    batch_vi = sum(n * _iterations(i) for n, i in zip(voxels, batch_infos))
+    diffs = [abs(b["final_max_error"] - p) for b, p in zip(batch_infos, plain_max)]
+    same_reason = sum(1 for b, p in zip(batch_infos, plain_infos) if b["reason"] == p["reason"])
+    return {
+        "workload": "%d gradient solves, each axis U{48..160}, spacing 1, smoothed log-normal target, "
+                    "ellipsoid mask, zero init, it_max=%d" % (len(vols), it_max),
+        "voxels": sum(voxels),
+        "plain": {"wall_s": plain_wall, "voxel_iters_per_s": plain_vi / plain_wall,
+                  "iterations": sum(_iterations(i) for i in plain_infos)},
+        "batched": {"wall_s": batch_wall, "voxel_iters_per_s": batch_vi / batch_wall,
+                    "iterations": sum(_iterations(i) for i in batch_infos),
+                    "chunks": stats["chunks"], "pass_launches": stats["passes"],
+                    "graph_builds": stats["graph_builds"], "pool_peak_bytes": stats["pool_peak"],
+                    "failed": sum(1 for i in range(len(vols)) if res[i][0] != 0)},
+        "speedup": (batch_vi / batch_wall) / (plain_vi / plain_wall),
+        "same_reason": "%d/%d" % (same_reason, len(vols)),
+        "max_abs_diff_final_max_error": max(diffs),
+    }
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def serve_bench(count, qps, it_max, graphs, seed=7):
+    """Open-loop serving stream: Poisson arrivals at `qps`, cube side
+    log-normal (median 96, sigma_ln 0.35, clipped to [32, 256]), 80 % gradient
+    and 20 % greedy requests, targets and masks of the 128^3 workload's
+    distribution.  Requests arrive as pinned host buffers; latency runs from
+    the scheduled arrival to the result being back in host memory.
+    plain: one request at a time (upload, dt_solve_device, download), FIFO.
+    batched: the batch engine with host staging, requests admitted as soon as
+    they arrive and fit (continuous batching)."""
+    import collections
+    import numpy as np
+    import torch
+    from disptools_amd import native
+    from disptools_amd.workloads import serving_stream, volume_case_torch
+
+    dev = torch.device("cuda:0")
+    reqs = serving_stream(count, qps, seed=seed)
+    data = []
+    for r in reqs:
+        side = r["side"]
+        T, M = volume_case_torch((side, side, side), r["seed"], dev)
+        data.append((T.cpu().pin_memory(), M.cpu().pin_memory(),
+                     torch.zeros((3, side, side, side), dtype=torch.float32).pin_memory()))
+    torch.cuda.synchronize()
+    voxels = [r["side"] ** 3 for r in reqs]
+    nmax = max(voxels)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    # plain: FIFO, one solve at a time through device staging buffers
+    dT = torch.empty(nmax, dtype=torch.float32, device=dev)
+    dM = torch.empty(nmax, dtype=torch.uint8, device=dev)
+    dF = torch.empty(3 * nmax, dtype=torch.float32, device=dev)
+    lat_plain, vi_plain, info_plain = [], 0, []
+    start = time.perf_counter()
+    for i, r in enumerate(reqs):
+        now = time.perf_counter() - start
+        if now < r["arrival"]:
+            time.sleep(r["arrival"] - now)
+        T, M, F = data[i]
+        n, side = voxels[i], r["side"]
+        dT[:n].copy_(T.view(-1), non_blocking=True)
+        dM[:n].copy_(M.view(-1), non_blocking=True)
+        dF[:3 * n].zero_()
+        info, _ = native.solve_device(r["algorithm"], side, side, side, (1.0, 1.0, 1.0), dT.data_ptr(),
+                                      dM.data_ptr(), dF.data_ptr(), stream=sh, it_max=it_max)
+        F.view(-1).copy_(dF[:3 * n], non_blocking=True)
+        stream.synchronize()
+        lat_plain.append(time.perf_counter() - start - r["arrival"])
+        vi_plain += n * _iterations(info)
+        info_plain.append(info)
+    plain_span = time.perf_counter() - start - reqs[0]["arrival"]
+    del dT, dM, dF
+
+    # batched: continuous batching
+    lat_b = [None] * len(reqs)
+    info_b = [None] * len(reqs)
+    vi_b = 0
+    for d in data:
+        d[2].zero_()
+    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+        queue = collections.deque()
+        tickets = {}
+        nxt = 0
+        done = 0
+        start = time.perf_counter()
+        while done < len(reqs):
+            now = time.perf_counter() - start
+            while nxt < len(reqs) and reqs[nxt]["arrival"] <= now:
+                queue.append(nxt)
+                nxt += 1
+            while queue:
+                i = queue[0]
+                side = reqs[i]["side"]
+                T, M, F = data[i]
+                t = eng.submit(reqs[i]["algorithm"], side, side, side, (1.0, 1.0, 1.0), T.data_ptr(),
+                               M.data_ptr(), F.data_ptr(), host_io=True, it_max=it_max)
+                if t is None:
+                    break
+                tickets[t] = queue.popleft()
+            left = eng.progress(wait=True)
+            for t, status, info in eng.poll():
+                i = tickets.pop(t)
+                lat_b[i] = time.perf_counter() - start - reqs[i]["arrival"]
+                info_b[i] = info
+                vi_b += voxels[i] * _iterations(info)
+                done += 1
+            if left == 0 and not queue and nxt < len(reqs):
+                wait = reqs[nxt]["arrival"] - (time.perf_counter() - start)
+                if wait > 0:
+                    time.sleep(min(wait, 0.001))
+        batch_span = time.perf_counter() - start - reqs[0]["arrival"]
+        stats = eng.stats()
+    same_reason = sum(1 for a, b in zip(info_plain, info_b) if a["reason"] == b["reason"])
+    diff = max(abs(a["final_max_error"] - b["final_max_error"]) for a, b in zip(info_plain, info_b))
+    return {
+        "workload": "%d requests, Poisson %.1f req/s, cube side log-normal (median 96, sigma_ln 0.35, "
+                    "clip [32, 256]), 80%% gradient / 20%% greedy, it_max=%d, host buffers (pinned)"
+                    % (len(reqs), qps, it_max),
+        "voxels": sum(voxels),
+        "plain": {"p50_s": _pct(lat_plain, 50), "p99_s": _pct(lat_plain, 99),
+                  "mean_s": float(np.mean(lat_plain)), "voxel_iters_per_s": vi_plain / plain_span},
+        "batched": {"p50_s": _pct(lat_b, 50), "p99_s": _pct(lat_b, 99), "mean_s": float(np.mean(lat_b)),
+                    "voxel_iters_per_s": vi_b / batch_span, "chunks": stats["chunks"],
+                    "graph_builds": stats["graph_builds"], "pool_peak_bytes": stats["pool_peak"]},
+        "same_reason": "%d/%d" % (same_reason, len(reqs)),
+        "max_abs_diff_final_max_error": diff,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -104,6 +319,15 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         copy_calibration()
+        return
+    if args.mode in ("batch", "serve"):
+        from disptools_amd import native
+        if not native.set_device(0):
+            raise RuntimeError("no HIP device")
+        if args.mode == "batch":
+            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs)))
+        else:
+            print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs)))
         return
 
     import numpy as np
@@ -157,6 +381,9 @@ def main():
     avg_launch_s = (gpu_ms / 1e3) / launches
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
+    batch = batch_bench(args.batch, args.it_max, args.graphs) if args.batch > 0 else None
+    serving = serve_bench(args.serve, args.qps, args.it_max, args.graphs) if args.serve > 0 else None
+
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     cpu = cpu_baseline(T, M, sp, args.cpu_iters, threads) if args.cpu_iters > 0 else None
 
@@ -184,6 +411,8 @@ def main():
                      "kernel": "k_grad_pass", "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes / launches},
         "cpu_baseline": cpu,
+        "batch": batch,
+        "serving": serving,
         "solve": {"iterations": last["iterations"], "reason": native.REASONS[last["reason"]],
                   "evaluations": last["evaluations"], "launches": last["launches"],
                   "rejections": last["rejections"], "final_error": last["final_error"],

@@ -1,0 +1,688 @@
+// dt_engine.cpp -- continuous batching of solves of uneven sizes.
+//
+// Every admitted instance (a volume with its parameters) owns a slot: an
+// Inst record and a control block in slot-indexed device arrays, and a range
+// of a device pool holding its solver state.  A pass is one launch of the
+// fused gradient kernel over the tile table of all gradient instances, plus
+// the greedy step and evaluation over the table of all greedy instances;
+// each instance's last block runs that instance's controller, so step
+// lengths, line searches and stopping tests stay per instance and never wait
+// for the host.  The host enqueues chunks of passes (two in flight), reads
+// back the control blocks once per chunk, retires the instances that have
+// stopped (result copy, slot and memory freed) and admits new ones between
+// chunks; the tile tables are rebuilt whenever the set of instances changes.
+// A chunk can be replayed from a captured hipGraph.
+//
+// An instance's result does not depend on what else is in the batch: its
+// tiles, the order of its partial sums and its controller are its own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "disptools_hip.h"
+#include "dt_host.hpp"
+
+namespace dt {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum SlotState { SLOT_FREE = 0, SLOT_ACTIVE = 1, SLOT_RETIRING = 2 };
+
+struct Slot {
+    int state = SLOT_FREE;
+    long long ticket = -1;
+    Plan P;
+    dt_params params{};
+    int host_io = 0;
+    float *out_field = nullptr;   // where the result goes (caller's buffer)
+    float *dev_field = nullptr;   // buf[0] of the instance
+    size_t pool_off = 0, pool_bytes = 0;
+    long long first_seq = 0;      // first chunk that includes this instance
+    long long launched = 0;       // passes launched over this instance
+    int failed = 0;
+    Ctrl final_ctrl{};
+    Clock::time_point t_submit;
+    hipEvent_t ev_ready = nullptr;   // inputs uploaded (host_io)
+    hipEvent_t ev_stop = nullptr;    // st reached the retirement point
+    hipEvent_t ev_done = nullptr;    // result in place
+};
+
+struct Chunk {
+    long long seq;
+    int snap;        // pinned snapshot buffer
+    int hi;          // slots [0, hi) were snapshotted
+    hipEvent_t ev;
+};
+
+// First-fit allocator over [0, size) with coalescing.
+struct Pool {
+    std::map<size_t, size_t> free_;   // offset -> bytes
+    size_t size = 0, used = 0, peak = 0;
+    void init(size_t s)
+    {
+        size = s;
+        free_.clear();
+        free_[0] = s;
+    }
+    bool alloc(size_t bytes, size_t &off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used += bytes;
+                peak = std::max(peak, used);
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+constexpr size_t POOL_GRAIN = (size_t)1 << 16;
+constexpr int MAX_SLOTS = 4096;
+
+}  // namespace
+
+}  // namespace dt
+
+using namespace dt;
+
+struct dt_engine {
+    int device = 0;
+    int precision = 32;
+    int variant = 1;
+    int chunk = 4;
+    int use_graphs = 0;
+    int tz_policy = 0;
+    int max_slots = 256;
+    hipStream_t st = nullptr, io = nullptr, cap = nullptr;
+    bool own_st = false;
+
+    char *pool_mem = nullptr;
+    Pool pool;
+
+    Inst *d_insts = nullptr;
+    Ctrl *d_ctrl = nullptr;
+    unsigned int *d_counter = nullptr;
+    Inst *h_insts = nullptr;     // pinned mirror, one per slot
+    Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks
+    Ctrl *h_snap[2] = {nullptr, nullptr};
+    int *h_zero = nullptr;       // pinned 0 (deactivates a stuck instance)
+
+    // tile tables: [0] gradient pass, [1] greedy kernels
+    Tile *d_tiles[2] = {nullptr, nullptr};
+    size_t d_cap[2] = {0, 0};
+    Tile *h_tiles[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [staging][kind]
+    size_t h_cap = 0;
+    int ntiles[2] = {0, 0};
+    bool dirty = true;
+
+    hipGraphExec_t graph = nullptr;
+    int graph_ntiles[2] = {-1, -1};
+    Tile *graph_tiles[2] = {nullptr, nullptr};
+
+    std::vector<Slot> slots;
+    std::deque<Chunk> inflight;
+    std::vector<hipEvent_t> chunk_events;   // free chunk events
+    std::vector<dt_engine_result> results;
+    long long seq = 0;
+    long long next_ticket = 0;
+    int n_active = 0, n_retiring = 0;
+    dt_engine_stats stats{};
+};
+
+namespace dt {
+namespace {
+
+int engine_tz(const dt_engine *e, int nz)
+{
+    if (e->tz_policy > 0) return std::min(e->tz_policy, nz);
+    if (e->tz_policy < 0) return 0;   // make_plan: as a lone solve
+    const int chunks = (nz + 15) / 16;
+    return (nz + chunks - 1) / chunks;
+}
+
+Plan engine_plan(const dt_engine *e, int algorithm, int nx, int ny, int nz)
+{
+    return make_plan(algorithm, nx, ny, nz, e->precision, e->variant, engine_tz(e, nz));
+}
+
+size_t io_bytes(long long n)
+{
+    return align_up(3 * (size_t)n * sizeof(float), 256) + align_up((size_t)n * sizeof(float), 256) +
+           align_up((size_t)n, 256);
+}
+
+size_t footprint(const dt_engine *e, const Plan &P, int host_io)
+{
+    (void)e;
+    const size_t b = (host_io ? io_bytes(P.n) : 0) + scratch_bytes(P, 0, false);
+    return align_up(b, POOL_GRAIN);
+}
+
+int ensure_tables(dt_engine *e, size_t need)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (e->d_cap[k] < need) {
+            // the current table may still be read by queued passes
+            DT_HIP(hipStreamSynchronize(e->st));
+            if (e->d_tiles[k]) DT_HIP(hipFree(e->d_tiles[k]));
+            e->d_tiles[k] = nullptr;
+            const size_t cap = std::max<size_t>(need, 2 * e->d_cap[k]);
+            DT_HIP(hipMalloc(&e->d_tiles[k], cap * sizeof(Tile)));
+            e->d_cap[k] = cap;
+        }
+    }
+    if (e->h_cap < need) {
+        DT_HIP(hipStreamSynchronize(e->st));
+        const size_t cap = std::max<size_t>(need, 2 * e->h_cap);
+        for (int s = 0; s < 2; ++s)
+            for (int k = 0; k < 2; ++k) {
+                if (e->h_tiles[s][k]) DT_HIP(hipHostFree(e->h_tiles[s][k]));
+                e->h_tiles[s][k] = nullptr;
+                DT_HIP(hipHostMalloc((void **)&e->h_tiles[s][k], cap * sizeof(Tile), hipHostMallocDefault));
+            }
+        e->h_cap = cap;
+    }
+    return 0;
+}
+
+// Rebuilds and uploads the tile tables of the active instances (staging
+// buffer `stage` is free: the chunk that used it last has completed).
+int rebuild_tables(dt_engine *e, int stage)
+{
+    std::vector<std::pair<int, int>> seg[2];
+    size_t count[2] = {0, 0};
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
+        const Slot &sl = e->slots[s];
+        if (sl.state != SLOT_ACTIVE) continue;
+        const int k = sl.P.algorithm == DT_GRADIENT ? 0 : 1;
+        seg[k].push_back({s, sl.P.ntiles});
+        count[k] += (size_t)sl.P.ntiles;
+    }
+    if (ensure_tables(e, std::max<size_t>(std::max(count[0], count[1]), 1))) return -1;
+    for (int k = 0; k < 2; ++k) {
+        std::vector<Tile> t;
+        t.reserve(count[k]);
+        append_tiles_xcd(t, seg[k]);
+        e->ntiles[k] = (int)t.size();
+        if (!t.empty()) {
+            std::memcpy(e->h_tiles[stage][k], t.data(), t.size() * sizeof(Tile));
+            DT_HIP(hipMemcpyAsync(e->d_tiles[k], e->h_tiles[stage][k], t.size() * sizeof(Tile),
+                                  hipMemcpyHostToDevice, e->st));
+        }
+    }
+    e->dirty = false;
+    return 0;
+}
+
+int launch_passes(dt_engine *e, hipStream_t s)
+{
+    for (int i = 0; i < e->chunk; ++i) {
+        if (e->ntiles[0] > 0)
+            DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, e->d_tiles[0], e->ntiles[0], s));
+        if (e->ntiles[1] > 0) {
+            DT_HIP(launch_greedy_step(e->d_insts, e->d_tiles[1], e->ntiles[1], s));
+            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, e->d_tiles[1], e->ntiles[1], s));
+        }
+    }
+    return 0;
+}
+
+int build_graph(dt_engine *e)
+{
+    if (e->graph) {
+        DT_HIP(hipGraphExecDestroy(e->graph));
+        e->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    DT_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_passes(e, e->cap);
+    const hipError_t ce = hipStreamEndCapture(e->cap, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return -1;
+    }
+    DT_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    DT_HIP(ie);
+    e->graph_ntiles[0] = e->ntiles[0];
+    e->graph_ntiles[1] = e->ntiles[1];
+    e->graph_tiles[0] = e->d_tiles[0];
+    e->graph_tiles[1] = e->d_tiles[1];
+    e->stats.graph_builds++;
+    return 0;
+}
+
+int enqueue_chunk(dt_engine *e)
+{
+    const int stage = (int)(e->seq & 1);
+    if (e->dirty && rebuild_tables(e, stage)) return -1;
+    if (e->use_graphs) {
+        if (!e->graph || e->graph_ntiles[0] != e->ntiles[0] || e->graph_ntiles[1] != e->ntiles[1] ||
+            e->graph_tiles[0] != e->d_tiles[0] || e->graph_tiles[1] != e->d_tiles[1]) {
+            if (build_graph(e)) return -1;
+        }
+        DT_HIP(hipGraphLaunch(e->graph, e->st));
+    } else {
+        if (launch_passes(e, e->st)) return -1;
+    }
+    int hi = 0;
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
+        Slot &sl = e->slots[s];
+        if (sl.state != SLOT_ACTIVE) continue;
+        hi = s + 1;
+        sl.launched += e->chunk;
+    }
+    Chunk c;
+    c.seq = e->seq++;
+    c.snap = stage;
+    c.hi = hi;
+    if (e->chunk_events.empty()) {
+        hipEvent_t ev;
+        DT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->chunk_events.push_back(ev);
+    }
+    c.ev = e->chunk_events.back();
+    e->chunk_events.pop_back();
+    if (hi > 0)
+        DT_HIP(hipMemcpyAsync(e->h_snap[stage], e->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost,
+                              e->st));
+    DT_HIP(hipEventRecord(c.ev, e->st));
+    e->inflight.push_back(c);
+    e->stats.chunks++;
+    e->stats.passes += e->chunk;
+    return 0;
+}
+
+// The instance has stopped (or is declared stuck): queue its result copy.
+int retire(dt_engine *e, int s, const Ctrl &fin, int failed)
+{
+    Slot &sl = e->slots[s];
+    sl.final_ctrl = fin;
+    sl.failed = failed;
+    sl.state = SLOT_RETIRING;
+    e->n_active--;
+    e->n_retiring++;
+    e->dirty = true;
+    const int rb = (fin.result_buf >= 0 && fin.result_buf < NBUF) ? fin.result_buf : 0;
+    const size_t bytes = 3 * (size_t)sl.P.n * sizeof(float);
+    float *src = e->h_insts[s].buf[rb];
+    if (sl.host_io) {
+        DT_HIP(hipEventRecord(sl.ev_stop, e->st));
+        DT_HIP(hipStreamWaitEvent(e->io, sl.ev_stop, 0));
+        DT_HIP(hipMemcpyAsync(sl.out_field, src, bytes, hipMemcpyDeviceToHost, e->io));
+        DT_HIP(hipEventRecord(sl.ev_done, e->io));
+    } else {
+        if (src != sl.out_field)
+            DT_HIP(hipMemcpyAsync(sl.out_field, src, bytes, hipMemcpyDeviceToDevice, e->st));
+        DT_HIP(hipEventRecord(sl.ev_done, e->st));
+    }
+    return 0;
+}
+
+void finish(dt_engine *e, int s)
+{
+    Slot &sl = e->slots[s];
+    dt_engine_result r{};
+    r.ticket = sl.ticket;
+    r.status = sl.failed ? -1 : 0;
+    const Ctrl &f = sl.final_ctrl;
+    r.info.iterations = f.it;
+    r.info.evaluations = f.evaluations;
+    r.info.rejections = f.rejections;
+    r.info.launches = sl.launched;
+    r.info.reason = sl.failed ? DT_RUNNING : f.reason;
+    r.info.final_error = f.E_result;
+    r.info.final_max_error = f.m_result;
+    r.info.seconds = std::chrono::duration<double>(Clock::now() - sl.t_submit).count();
+    r.info.trace_len = 0;
+    r.info.field_io = f.field_io;
+    e->results.push_back(r);
+    e->pool.release(sl.pool_off, sl.pool_bytes);
+    sl.state = SLOT_FREE;
+    sl.ticket = -1;
+    e->n_retiring--;
+    e->stats.finished++;
+}
+
+int reap(dt_engine *e, bool wait)
+{
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
+        Slot &sl = e->slots[s];
+        if (sl.state != SLOT_RETIRING) continue;
+        hipError_t q = wait ? hipEventSynchronize(sl.ev_done) : hipEventQuery(sl.ev_done);
+        if (q == hipErrorNotReady) continue;
+        DT_HIP(q);
+        finish(e, s);
+    }
+    return 0;
+}
+
+int process_chunk(dt_engine *e, const Chunk &c)
+{
+    const Ctrl *snap = e->h_snap[c.snap];
+    for (int s = 0; s < c.hi; ++s) {
+        Slot &sl = e->slots[s];
+        if (sl.state != SLOT_ACTIVE || c.seq < sl.first_seq) continue;
+        if (snap[s].phase == PH_DONE) {
+            if (retire(e, s, snap[s], 0)) return -1;
+        } else if (sl.launched > max_passes(sl.params)) {
+            // stop its passes, then return the last accepted iterate
+            DT_HIP(hipMemcpyAsync(&e->d_ctrl[s].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, e->st));
+            Ctrl f = snap[s];
+            f.result_buf = f.base;
+            if (retire(e, s, f, 1)) return -1;
+        }
+    }
+    return 0;
+}
+
+int progress(dt_engine *e, int wait)
+{
+    if (reap(e, false)) return -1;
+    while ((int)e->inflight.size() < 2 && e->n_active > 0)
+        if (enqueue_chunk(e)) return -1;
+    if (!e->inflight.empty()) {
+        const Chunk c = e->inflight.front();
+        hipError_t q = wait ? hipEventSynchronize(c.ev) : hipEventQuery(c.ev);
+        if (q != hipErrorNotReady) {
+            DT_HIP(q);
+            e->inflight.pop_front();
+            e->chunk_events.push_back(c.ev);
+            if (process_chunk(e, c)) return -1;
+            // keep the device busy while the host handles results
+            while ((int)e->inflight.size() < 2 && e->n_active > 0)
+                if (enqueue_chunk(e)) return -1;
+        }
+    } else if (wait && e->n_retiring > 0) {
+        if (reap(e, true)) return -1;
+    }
+    if (reap(e, false)) return -1;
+    return e->n_active + e->n_retiring;
+}
+
+void destroy(dt_engine *e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->io) (void)hipStreamSynchronize(e->io);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
+    for (auto &c : e->inflight) (void)hipEventDestroy(c.ev);
+    for (auto ev : e->chunk_events) (void)hipEventDestroy(ev);
+    for (auto &sl : e->slots) {
+        if (sl.ev_ready) (void)hipEventDestroy(sl.ev_ready);
+        if (sl.ev_stop) (void)hipEventDestroy(sl.ev_stop);
+        if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
+    }
+    for (int k = 0; k < 2; ++k)
+        if (e->d_tiles[k]) (void)hipFree(e->d_tiles[k]);
+    for (int s = 0; s < 2; ++s)
+        for (int k = 0; k < 2; ++k)
+            if (e->h_tiles[s][k]) (void)hipHostFree(e->h_tiles[s][k]);
+    for (int s = 0; s < 2; ++s)
+        if (e->h_snap[s]) (void)hipHostFree(e->h_snap[s]);
+    if (e->h_insts) (void)hipHostFree(e->h_insts);
+    if (e->h_ctrl0) (void)hipHostFree(e->h_ctrl0);
+    if (e->h_zero) (void)hipHostFree(e->h_zero);
+    if (e->d_insts) (void)hipFree(e->d_insts);
+    if (e->d_ctrl) (void)hipFree(e->d_ctrl);
+    if (e->d_counter) (void)hipFree(e->d_counter);
+    if (e->pool_mem) (void)hipFree(e->pool_mem);
+    if (e->cap) (void)hipStreamDestroy(e->cap);
+    if (e->io) (void)hipStreamDestroy(e->io);
+    if (e->own_st && e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
+int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
+{
+    DT_HIP(hipGetDevice(&e->device));
+    e->precision = precision_from_env();
+    e->variant = (cfg && cfg->variant >= 0 && cfg->variant < GRAD_VARIANTS) ? cfg->variant : variant_from_env();
+    e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 4;
+    e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
+    e->tz_policy = cfg ? cfg->tz : 0;
+    e->max_slots = (cfg && cfg->max_slots > 0) ? cfg->max_slots : 256;
+    if (e->max_slots > MAX_SLOTS) DT_FAIL("max_slots is at most 4096");
+    size_t budget = cfg ? cfg->hbm_budget : 0;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        DT_HIP(hipMemGetInfo(&fr, &tot));
+        budget = fr / 2;
+    }
+    budget = budget / POOL_GRAIN * POOL_GRAIN;
+    if (budget < POOL_GRAIN) DT_FAIL("hbm_budget too small");
+
+    if (stream) {
+        e->st = (hipStream_t)stream;
+    } else {
+        DT_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+        e->own_st = true;
+    }
+    DT_HIP(hipStreamCreateWithFlags(&e->io, hipStreamNonBlocking));
+    DT_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
+
+    if (hipMalloc(&e->pool_mem, budget) != hipSuccess) {
+        (void)hipGetLastError();
+        e->pool_mem = nullptr;
+        DT_FAIL("cannot allocate the engine's device pool (hbm_budget)");
+    }
+    e->pool.init(budget);
+    const int S = e->max_slots;
+    DT_HIP(hipMalloc(&e->d_insts, (size_t)S * sizeof(Inst)));
+    DT_HIP(hipMalloc(&e->d_ctrl, (size_t)S * sizeof(Ctrl)));
+    DT_HIP(hipMalloc(&e->d_counter, (size_t)S * sizeof(unsigned int)));
+    DT_HIP(hipMemset(e->d_ctrl, 0, (size_t)S * sizeof(Ctrl)));
+    DT_HIP(hipMemset(e->d_counter, 0, (size_t)S * sizeof(unsigned int)));
+    DT_HIP(hipHostMalloc((void **)&e->h_insts, (size_t)S * sizeof(Inst), hipHostMallocDefault));
+    DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
+    DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
+    *e->h_zero = 0;
+    for (int s = 0; s < 2; ++s)
+        DT_HIP(hipHostMalloc((void **)&e->h_snap[s], (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
+    e->slots.resize(S);
+    for (auto &sl : e->slots) {
+        DT_HIP(hipEventCreateWithFlags(&sl.ev_ready, hipEventDisableTiming));
+        DT_HIP(hipEventCreateWithFlags(&sl.ev_stop, hipEventDisableTiming));
+        DT_HIP(hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming));
+    }
+    e->stats.pool_bytes = budget;
+    return 0;
+}
+
+long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                 const float *target, const unsigned char *mask, const dt_params *params, float *field,
+                 int host_io)
+{
+    if (validate(algorithm, nx, ny, nz, dx, dy, dz, params)) return -1;
+    if (!target || !mask || !field) DT_FAIL("null buffer");
+    int s = -1;
+    for (int i = 0; i < (int)e->slots.size(); ++i)
+        if (e->slots[i].state == SLOT_FREE) { s = i; break; }
+    if (s < 0) return DT_ENGINE_FULL;
+    const Plan P = engine_plan(e, algorithm, (int)nx, (int)ny, (int)nz);
+    const size_t bytes = footprint(e, P, host_io);
+    if (bytes > e->pool.size) DT_FAIL("instance larger than the engine's hbm_budget");
+    size_t off = 0;
+    if (!e->pool.alloc(bytes, off)) return DT_ENGINE_FULL;
+
+    Slot &sl = e->slots[s];
+    sl.P = P;
+    sl.params = *params;
+    sl.host_io = host_io;
+    sl.pool_off = off;
+    sl.pool_bytes = bytes;
+    sl.failed = 0;
+    sl.t_submit = Clock::now();
+    const long long n = P.n;
+    Carver cv(e->pool_mem + off);
+    const float *d_target;
+    const uint8_t *d_mask;
+    float *d_field;
+    int rc = 0;
+    if (host_io) {
+        d_field = cv.take<float>(3 * (size_t)n);
+        float *t = cv.take<float>((size_t)n);
+        uint8_t *m = cv.take<uint8_t>((size_t)n);
+        if (hipMemcpyAsync(d_field, field, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, e->io) !=
+                hipSuccess ||
+            hipMemcpyAsync(t, target, (size_t)n * sizeof(float), hipMemcpyHostToDevice, e->io) != hipSuccess ||
+            hipMemcpyAsync(m, mask, (size_t)n, hipMemcpyHostToDevice, e->io) != hipSuccess ||
+            hipEventRecord(sl.ev_ready, e->io) != hipSuccess ||
+            hipStreamWaitEvent(e->st, sl.ev_ready, 0) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot stage the instance's inputs on the device");
+            rc = -1;
+        }
+        d_target = t;
+        d_mask = m;
+    } else {
+        d_field = field;
+        d_target = target;
+        d_mask = (const uint8_t *)mask;
+    }
+    sl.out_field = field;
+    sl.dev_field = d_field;
+    if (!rc) {
+        e->h_insts[s] = make_inst(P, dx, dy, dz, *params, d_target, d_mask, d_field, cv, nullptr, 0,
+                                  &e->d_ctrl[s], &e->d_counter[s]);
+        e->h_ctrl0[s] = initial_ctrl(algorithm, *params);
+        if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, e->st) !=
+                hipSuccess ||
+            hipMemcpyAsync(&e->d_ctrl[s], &e->h_ctrl0[s], sizeof(Ctrl), hipMemcpyHostToDevice, e->st) !=
+                hipSuccess ||
+            hipMemsetAsync(&e->d_counter[s], 0, sizeof(unsigned int), e->st) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot upload the instance record");
+            rc = -1;
+        }
+        // the greedy step never writes the outer face of its output buffer,
+        // which starts at zero (src/disptools.c:61, displacement_field_greedy.c:43-65)
+        if (!rc && algorithm == DT_GREEDY &&
+            hipMemsetAsync(e->h_insts[s].buf[1], 0, 3 * (size_t)n * sizeof(float), e->st) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot clear the greedy buffer");
+            rc = -1;
+        }
+    }
+    if (rc) {
+        e->pool.release(off, bytes);
+        return -1;
+    }
+    sl.state = SLOT_ACTIVE;
+    sl.ticket = e->next_ticket++;
+    sl.first_seq = e->seq;
+    sl.launched = 0;
+    e->n_active++;
+    e->dirty = true;
+    e->stats.admitted++;
+    return sl.ticket;
+}
+
+}  // namespace
+}  // namespace dt
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" dt_engine *dt_engine_create(const dt_engine_config *cfg, void *stream)
+{
+    disptools_clear_error();
+    dt_engine *e = new (std::nothrow) dt_engine();
+    if (!e) {
+        set_error(__LINE__, __FILE__, __func__, "out of host memory");
+        return nullptr;
+    }
+    if (create(cfg, stream, e)) {
+        destroy(e);
+        return nullptr;
+    }
+    return e;
+}
+
+extern "C" void dt_engine_destroy(dt_engine *e) { destroy(e); }
+
+extern "C" size_t dt_engine_footprint(const dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz,
+                                      int host_io)
+{
+    if (!e || nx < 3 || ny < 3 || nz < 3) return 0;
+    (void)hipSetDevice(e->device);
+    return footprint(e, engine_plan(e, algorithm, (int)nx, (int)ny, (int)nz), host_io);
+}
+
+extern "C" long long dt_engine_submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, float dx,
+                                      float dy, float dz, const float *target, const unsigned char *mask,
+                                      const dt_params *params, float *field, int host_io)
+{
+    disptools_clear_error();
+    if (!e) DT_FAIL("null engine");
+    DT_HIP(hipSetDevice(e->device));
+    return submit(e, algorithm, nx, ny, nz, dx, dy, dz, target, mask, params, field, host_io ? 1 : 0);
+}
+
+extern "C" int dt_engine_progress(dt_engine *e, int wait)
+{
+    if (!e) DT_FAIL("null engine");
+    DT_HIP(hipSetDevice(e->device));
+    return progress(e, wait);
+}
+
+extern "C" int dt_engine_poll(dt_engine *e, dt_engine_result *out, int cap)
+{
+    if (!e || !out || cap <= 0) return 0;
+    const int k = std::min<int>(cap, (int)e->results.size());
+    std::copy(e->results.begin(), e->results.begin() + k, out);
+    e->results.erase(e->results.begin(), e->results.begin() + k);
+    return k;
+}
+
+extern "C" int dt_engine_drain(dt_engine *e)
+{
+    if (!e) DT_FAIL("null engine");
+    DT_HIP(hipSetDevice(e->device));
+    for (;;) {
+        const int left = progress(e, 1);
+        if (left < 0) return -1;
+        if (left == 0) return 0;
+    }
+}
+
+extern "C" int dt_engine_get_stats(const dt_engine *e, dt_engine_stats *out)
+{
+    if (!e || !out) return -1;
+    *out = e->stats;
+    out->active = e->n_active;
+    out->retiring = e->n_retiring;
+    out->pool_used = e->pool.used;
+    out->pool_peak = e->pool.peak;
+    return 0;
+}

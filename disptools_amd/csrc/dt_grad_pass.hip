@@ -435,6 +435,7 @@ static int occupancy_ct(int variant)
 hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                             hipStream_t st)
 {
+    if (variant >= STREAM_BASE) return launch_stream_pass(precision, variant, insts, tiles, nblocks, st);
     if (variant >= SPLIT_BASE) return launch_split_pass(precision, variant, insts, tiles, nblocks, st);
     return precision == 64 ? launch_ct<double>(variant, insts, tiles, nblocks, st)
                            : launch_ct<float>(variant, insts, tiles, nblocks, st);
@@ -442,6 +443,7 @@ hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const
 
 int grad_pass_blocks_per_cu(int precision, int variant)
 {
+    if (variant >= STREAM_BASE) return stream_pass_blocks_per_cu(precision, variant);
     if (variant >= SPLIT_BASE) return split_pass_blocks_per_cu(precision, variant);
     return precision == 64 ? occupancy_ct<double>(variant) : occupancy_ct<float>(variant);
 }

@@ -29,6 +29,7 @@ __global__ void __launch_bounds__(TX * TY)
 k_greedy_step(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
 {
     const Tile tl = tiles[blockIdx.x];
+    if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
     const Ctrl *C = I.ctrl;
     if (!C->active || !C->step_on) return;
@@ -92,6 +93,7 @@ k_greedy_eval(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
     __shared__ double red[3 * (NT / 64)];
     __shared__ int flag;
     const Tile tl = tiles[blockIdx.x];
+    if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
     Ctrl *C = I.ctrl;
     if (!C->active) return;

@@ -17,12 +17,18 @@ hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st);
 // Variants of the fused gradient pass: column tile TX x TY (one thread per
 // column) and the waves per SIMD its registers are capped for (3: may spill a
 // few registers; 2: no spill, lower occupancy; 4: spills more).
-// Variants from SPLIT_BASE on run each pass as two kernels (dt_split_pass.hip).
-constexpr int GRAD_VARIANTS = 11;
+// Variants SPLIT_BASE .. SPLIT_BASE+2 run each pass as two kernels
+// (dt_split_pass.hip); variants from STREAM_BASE on are the single-barrier
+// streaming form (dt_grad_stream.hip).
+constexpr int GRAD_VARIANTS = 15;
 constexpr int SPLIT_BASE = 8;
+constexpr int STREAM_BASE = 11;
 inline void grad_variant_dims(int v, int &tx, int &ty)
 {
     switch (v) {
+    case STREAM_BASE: case STREAM_BASE + 3: tx = 32; ty = 16; break;
+    case STREAM_BASE + 1: tx = 32; ty = 8; break;
+    case STREAM_BASE + 2: tx = 64; ty = 8; break;
     case SPLIT_BASE: tx = 32; ty = 16; break;
     case SPLIT_BASE + 1: tx = 32; ty = 8; break;
     case SPLIT_BASE + 2: tx = 64; ty = 8; break;
@@ -40,6 +46,9 @@ int grad_pass_blocks_per_cu(int precision, int variant);
 hipError_t launch_split_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                              hipStream_t st);
 int split_pass_blocks_per_cu(int precision, int variant);
+hipError_t launch_stream_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
+                              hipStream_t st);
+int stream_pass_blocks_per_cu(int precision, int variant);
 hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 

@@ -107,7 +107,7 @@ size_t scratch_bytes(const Plan &P, long long trace_cap, bool own_ctrl)
     const int extra_fields = P.algorithm == DT_GRADIENT ? NBUF - 1 : 1;
     for (int i = 0; i < extra_fields; ++i) add(3 * (size_t)n * sizeof(float));
     if (P.algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
-    if (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE) add((size_t)n * sizeof(float));
+    if (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE && P.variant < STREAM_BASE) add((size_t)n * sizeof(float));
     add((size_t)P.ntiles * 4 * sizeof(double));
     if (trace_cap > 0) add((size_t)trace_cap * 4 * sizeof(double));
     if (own_ctrl) { add(sizeof(unsigned int)); add(sizeof(Ctrl)); }
@@ -139,7 +139,7 @@ Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, 
         h.err[0] = cv.take<float>((size_t)n);
         h.err[1] = cv.take<float>((size_t)n);
     }
-    h.wmap = (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE) ? cv.take<float>((size_t)n) : nullptr;
+    h.wmap = (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE && P.variant < STREAM_BASE) ? cv.take<float>((size_t)n) : nullptr;
     h.partials = cv.take<double>((size_t)P.ntiles * 4);
     if (trace_cap > 0) {
         double *t = cv.take<double>((size_t)trace_cap * 4);

@@ -18,6 +18,15 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "disptools_hip.h")
 if not os.path.exists(LIB_PATH):
     raise ImportError("%s is not built; run `python -m disptools_amd.build`" % LIB_PATH)
 
+# PyTorch-ROCm ships its own HIP runtime (same soname as /opt/rocm's).  Loaded
+# first, it is the one runtime of the process and this library binds to it,
+# so device pointers of torch tensors are valid here; loaded after this
+# library, torch would find the other runtime already in place and no device.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = ctypes.CDLL(LIB_PATH)
 
 GRADIENT = 0
@@ -106,6 +115,60 @@ lib.dt_solve_host.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp,
 lib.dt_solve_host.restype = ctypes.c_int
 
 
+class EngineConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_slots", ctypes.c_int),
+        ("chunk", ctypes.c_int),
+        ("use_graphs", ctypes.c_int),
+        ("variant", ctypes.c_int),
+        ("tz", ctypes.c_int),
+    ]
+
+
+class EngineResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_longlong), ("status", ctypes.c_int), ("info", SolveInfo)]
+
+
+class EngineStats(ctypes.Structure):
+    _fields_ = [
+        ("chunks", ctypes.c_longlong),
+        ("passes", ctypes.c_longlong),
+        ("graph_builds", ctypes.c_longlong),
+        ("admitted", ctypes.c_longlong),
+        ("finished", ctypes.c_longlong),
+        ("active", ctypes.c_int),
+        ("retiring", ctypes.c_int),
+        ("pool_bytes", ctypes.c_size_t),
+        ("pool_used", ctypes.c_size_t),
+        ("pool_peak", ctypes.c_size_t),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+ENGINE_FULL = -2
+
+lib.dt_engine_create.argtypes = [ctypes.POINTER(EngineConfig), _vp]
+lib.dt_engine_create.restype = _vp
+lib.dt_engine_destroy.argtypes = [_vp]
+lib.dt_engine_destroy.restype = None
+lib.dt_engine_footprint.argtypes = [_vp, ctypes.c_int, _sz, _sz, _sz, ctypes.c_int]
+lib.dt_engine_footprint.restype = _sz
+lib.dt_engine_submit.argtypes = [_vp, ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp,
+                                 ctypes.POINTER(Params), _vp, ctypes.c_int]
+lib.dt_engine_submit.restype = ctypes.c_longlong
+lib.dt_engine_progress.argtypes = [_vp, ctypes.c_int]
+lib.dt_engine_progress.restype = ctypes.c_int
+lib.dt_engine_poll.argtypes = [_vp, ctypes.POINTER(EngineResult), ctypes.c_int]
+lib.dt_engine_poll.restype = ctypes.c_int
+lib.dt_engine_drain.argtypes = [_vp]
+lib.dt_engine_drain.restype = ctypes.c_int
+lib.dt_engine_get_stats.argtypes = [_vp, ctypes.POINTER(EngineStats)]
+lib.dt_engine_get_stats.restype = ctypes.c_int
+
+
 class HipError(RuntimeError):
     pass
 
@@ -124,6 +187,7 @@ def declared_symbols():
     """Function names declared in include/disptools_hip.h."""
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"^\s*#.*$", "", text, flags=re.M)
     names = re.findall(r"\b([A-Za-z_]\w*)\s*\(", text)
     keywords = {"if", "while", "for", "return", "sizeof", "defined"}
     return sorted({n for n in names if n not in keywords})
@@ -199,3 +263,78 @@ def jacobian_device(nx, ny, nz, spacing, d_field, d_J, stream=0):
     rc = lib.dt_jacobian_device(nx, ny, nz, *map(float, spacing), d_field, d_J, stream or None)
     if rc != 0:
         raise HipError(last_error())
+
+
+class Engine:
+    """Continuous-batching solver (``dt_engine_*``).
+
+    ``submit`` admits a solve if a slot and pool memory are free (returns a
+    ticket) or returns None (full: call ``progress`` and retry); ``progress``
+    advances the passes; ``poll`` returns finished ``(ticket, status, info)``.
+    Buffers are device pointers (host_io=False) or host pointers (True).
+    """
+
+    def __init__(self, hbm_budget=0, max_slots=0, chunk=0, use_graphs=False, variant=-1, tz=0,
+                 stream=0):
+        cfg = EngineConfig(int(hbm_budget), int(max_slots), int(chunk), 1 if use_graphs else 0,
+                           int(variant), int(tz))
+        h = lib.dt_engine_create(ctypes.byref(cfg), stream or None)
+        if not h:
+            raise HipError(last_error())
+        self._h = h
+        self._buf = (EngineResult * 64)()
+
+    def close(self):
+        if self._h:
+            lib.dt_engine_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self, algorithm, nx, ny, nz, host_io=False):
+        return int(lib.dt_engine_footprint(self._h, ALGORITHMS[algorithm], nx, ny, nz, 1 if host_io else 0))
+
+    def submit(self, algorithm, nx, ny, nz, spacing, target, mask, field, host_io=False, **params):
+        p = make_params(**params)
+        t = lib.dt_engine_submit(self._h, ALGORITHMS[algorithm], nx, ny, nz, *map(float, spacing),
+                                 target, mask, ctypes.byref(p), field, 1 if host_io else 0)
+        if t == ENGINE_FULL:
+            return None
+        if t < 0:
+            raise HipError(last_error())
+        return int(t)
+
+    def progress(self, wait=True):
+        r = lib.dt_engine_progress(self._h, 1 if wait else 0)
+        if r < 0:
+            raise HipError(last_error())
+        return r
+
+    def poll(self):
+        out = []
+        while True:
+            k = lib.dt_engine_poll(self._h, self._buf, len(self._buf))
+            for i in range(k):
+                r = self._buf[i]
+                out.append((int(r.ticket), int(r.status), r.info.as_dict()))
+            if k < len(self._buf):
+                return out
+
+    def drain(self):
+        if lib.dt_engine_drain(self._h) != 0:
+            raise HipError(last_error())
+
+    def stats(self):
+        s = EngineStats()
+        lib.dt_engine_get_stats(self._h, ctypes.byref(s))
+        return s.as_dict()

@@ -17,6 +17,10 @@ Volumes are numpy arrays indexed [z, y, x]; spacing tuples are (dx, dy, dz).
                           smoothing 6, clip [0.7, 1.4], two spheres of radii 40
                           and 60 centred at (x, y, z) = (96, 128, 128), (160, 128, 128).
 * ``serving_stream``   -- Poisson arrivals with log-normal cube sides.
+* ``volume_case_torch`` -- the ``volume_case`` distribution drawn on a torch
+                          device (noise from torch's generator, so the values
+                          differ from ``volume_case`` for the same seed): used
+                          where hundreds of volumes are needed quickly.
 """
 
 import math
@@ -109,3 +113,52 @@ def serving_stream(count, qps, seed=7, median=96, sigma_ln=0.35, lo=32, hi=256, 
         alg = "greedy" if rng.random() < greedy_frac else "gradient"
         out.append(dict(arrival=t, side=side, algorithm=alg, seed=int(seed * 100000 + i)))
     return out
+
+
+def _gauss_kernel(sigma, truncate=4.0):
+    r = int(truncate * sigma + 0.5)
+    x = np.arange(-r, r + 1, dtype=np.float64)
+    k = np.exp(-0.5 * (x / sigma) ** 2)
+    return (k / k.sum()).astype(np.float32), r
+
+
+def volume_case_torch(shape, seed, device, sigma_g=0.05, smooth=4.0, clip=(0.8, 1.25)):
+    """Target and mask of ``volume_case``'s distribution, built on `device`:
+    separable Gaussian smoothing with edge replication ("nearest"), ellipsoid
+    parameters from numpy's PCG64 (seed), membership evaluated on the device."""
+    import torch
+    import torch.nn.functional as F
+    nz, ny, nx = shape
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    g = torch.randn(shape, generator=gen, device=device, dtype=torch.float32) * sigma_g
+    k, r = _gauss_kernel(smooth)
+    kt = torch.from_numpy(k).to(device)
+    v = g[None, None]
+    for axis in range(3):
+        pad = [0, 0, 0, 0, 0, 0]
+        pad[2 * (2 - axis)] = r
+        pad[2 * (2 - axis) + 1] = r
+        v = F.pad(v, pad, mode="replicate")
+        w = kt.view([1, 1] + [len(k) if a == axis else 1 for a in range(3)])
+        v = F.conv3d(v, w)
+    target = torch.clamp(torch.exp(v[0, 0]), clip[0], clip[1]).contiguous()
+
+    rng = np.random.default_rng(seed)
+    z = torch.arange(nz, device=device, dtype=torch.float32).view(nz, 1, 1)
+    y = torch.arange(ny, device=device, dtype=torch.float32).view(1, ny, 1)
+    x = torch.arange(nx, device=device, dtype=torch.float32).view(1, 1, nx)
+    mask = torch.zeros(shape, dtype=torch.bool, device=device)
+    kcount = int(rng.integers(3, 9))
+    n = np.array([nx, ny, nz], np.float64)
+    for _ in range(kcount):
+        semi = rng.uniform(8.0, 30.0, size=3)
+        centre = rng.uniform(0.2 * n, 0.8 * n)
+        rot = _haar_rotation(rng)
+        px, py, pz = x - float(centre[0]), y - float(centre[1]), z - float(centre[2])
+        acc = 0.0
+        for j in range(3):
+            lj = px * float(rot[0, j]) + py * float(rot[1, j]) + pz * float(rot[2, j])
+            acc = acc + (lj / float(semi[j])) ** 2
+        mask |= acc <= 1.0
+    return target, mask.to(torch.uint8).contiguous()

@@ -143,6 +143,82 @@ int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
                   const dt_params *params, float *field,
                   dt_solve_info *info, double *trace, long long trace_cap);
 
+/* ------------------------------------------------------------------------ */
+/* Batch engine: continuous batching of solves of uneven sizes.             */
+/* No counterpart in the reference (one solve per call there); for callers  */
+/* that serve many volumes: every pass launch covers all admitted volumes,  */
+/* each with its own control block (step length, line-search state,         */
+/* stopping tests) on the device.  Admission is by device-memory footprint. */
+/* One engine is driven from one host thread.                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct dt_engine dt_engine;
+
+typedef struct dt_engine_config {
+    size_t hbm_budget;   /* device bytes for instance state (0: half of the free memory) */
+    int max_slots;       /* instances on the device at once (0: 256; at most 4096)       */
+    int chunk;           /* passes enqueued per chunk (0: 4)                               */
+    int use_graphs;      /* 1: each chunk is replayed from a captured hipGraph             */
+    int variant;         /* gradient pass variant (-1: the library default)                */
+    int tz;              /* tile depth in planes: 0 = batch default (balanced, <= 16),
+                            -1 = as a lone dt_solve_device call, > 0 = fixed              */
+} dt_engine_config;
+
+typedef struct dt_engine_result {
+    long long ticket;
+    int status;          /* 0: solved; -1: failed (the line search did not terminate) */
+    dt_solve_info info;  /* seconds = from submission to the result being in place    */
+} dt_engine_result;
+
+typedef struct dt_engine_stats {
+    long long chunks;          /* chunks enqueued                          */
+    long long passes;          /* pass launches (per kernel kind)          */
+    long long graph_builds;    /* hipGraph captures + instantiations       */
+    long long admitted;        /* instances admitted so far                */
+    long long finished;        /* results produced so far                  */
+    int active;                /* instances being solved                   */
+    int retiring;              /* instances whose result copy is in flight */
+    size_t pool_bytes;         /* size of the device pool                  */
+    size_t pool_used;          /* bytes held by instances now              */
+    size_t pool_peak;          /* largest pool_used seen                   */
+} dt_engine_stats;
+
+/* Returned by dt_engine_submit when the instance does not fit now (no free
+ * slot or not enough pool memory): retry after results have retired. */
+#define DT_ENGINE_FULL (-2)
+
+/* Creates an engine on the current device.  `stream` (hipStream_t as void*,
+ * NULL: a stream of its own) carries the passes.  NULL on error. */
+dt_engine *dt_engine_create(const dt_engine_config *cfg, void *stream);
+void dt_engine_destroy(dt_engine *e);
+
+/* Pool bytes an instance takes (host_io: target, mask and field are staged
+ * in the pool too). */
+size_t dt_engine_footprint(const dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, int host_io);
+
+/* Admits a solve.  host_io = 0: target/mask/field are device buffers that
+ * stay valid and untouched until the result is polled; the result is written
+ * to `field`.  host_io = 1: host buffers (pinned memory lets the copies
+ * overlap the passes); inputs are copied at submission and the result is
+ * copied into `field` before the ticket is reported.  Returns a ticket >= 0,
+ * DT_ENGINE_FULL, or -1 (error state set). */
+long long dt_engine_submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz,
+                           float dx, float dy, float dz, const float *target, const unsigned char *mask,
+                           const dt_params *params, float *field, int host_io);
+
+/* Advances the engine: keeps two chunks of passes in flight, retires
+ * finished instances.  wait = 1 blocks until the oldest chunk has finished.
+ * Returns the number of instances not yet reported, or -1 on error. */
+int dt_engine_progress(dt_engine *e, int wait);
+
+/* Moves up to `cap` finished results to `out`; returns how many. */
+int dt_engine_poll(dt_engine *e, dt_engine_result *out, int cap);
+
+/* Progresses until every admitted instance has finished.  0 or -1. */
+int dt_engine_drain(dt_engine *e);
+
+int dt_engine_get_stats(const dt_engine *e, dt_engine_stats *out);
+
 #ifdef __cplusplus
 }
 #endif
