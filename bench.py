@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--serve", type=int, default=0,
                     help="requests of the serving stream (0 to skip it)")
     ap.add_argument("--qps", type=float, default=40.0)
-    ap.add_argument("--graphs", type=int, default=1, help="engine replays chunks from hipGraphs")
+    ap.add_argument("--graphs", type=int, default=0, help="engine replays chunks from hipGraphs")
     return ap.parse_args()
 
 
@@ -234,7 +234,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
     dT = torch.empty(nmax, dtype=torch.float32, device=dev)
     dM = torch.empty(nmax, dtype=torch.uint8, device=dev)
     dF = torch.empty(3 * nmax, dtype=torch.float32, device=dev)
-    lat_plain, vi_plain, info_plain = [], 0, []
+    lat_plain, vi_plain, info_plain, busy = [], 0, [], 0.0
     start = time.perf_counter()
     for i, r in enumerate(reqs):
         now = time.perf_counter() - start
@@ -242,6 +242,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
             time.sleep(r["arrival"] - now)
         T, M, F = data[i]
         n, side = voxels[i], r["side"]
+        t_begin = time.perf_counter()
         dT[:n].copy_(T.view(-1), non_blocking=True)
         dM[:n].copy_(M.view(-1), non_blocking=True)
         dF[:3 * n].zero_()
@@ -249,6 +250,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
                                       dM.data_ptr(), dF.data_ptr(), stream=sh, it_max=it_max)
         F.view(-1).copy_(dF[:3 * n], non_blocking=True)
         stream.synchronize()
+        busy += time.perf_counter() - t_begin
         lat_plain.append(time.perf_counter() - start - r["arrival"])
         vi_plain += n * _iterations(info)
         info_plain.append(info)
@@ -302,7 +304,8 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
                     % (len(reqs), qps, it_max),
         "voxels": sum(voxels),
         "plain": {"p50_s": _pct(lat_plain, 50), "p99_s": _pct(lat_plain, 99),
-                  "mean_s": float(np.mean(lat_plain)), "voxel_iters_per_s": vi_plain / plain_span},
+                  "mean_s": float(np.mean(lat_plain)), "voxel_iters_per_s": vi_plain / plain_span,
+                  "busy_fraction": busy / plain_span},
         "batched": {"p50_s": _pct(lat_b, 50), "p99_s": _pct(lat_b, 99), "mean_s": float(np.mean(lat_b)),
                     "voxel_iters_per_s": vi_b / batch_span, "chunks": stats["chunks"],
                     "graph_builds": stats["graph_builds"], "pool_peak_bytes": stats["pool_peak"]},

@@ -38,9 +38,25 @@ HIP_FLAGS = [
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_grad_pass.hip", "-ffp-contract=fast"),
                ("dt_split_pass.hip", "-ffp-contract=fast"),
-               ("dt_grad_stream.hip", "-ffp-contract=fast"),
                ("dt_runtime.cpp", "-ffp-contract=off"),
                ("dt_engine.cpp", "-ffp-contract=off")]
+
+
+# The streaming pass is compiled once per (precision, tile) variant, each
+# kernel in a translation unit of its own.
+# (tile x, tile y, waves per SIMD the registers are capped for)
+STREAM_VARIANTS = [(ct, tx, ty, w) for ct in ("float", "double")
+                   for tx, ty, w in ((32, 16, 3), (32, 20, 3), (64, 8, 3), (32, 24, 3), (64, 12, 3))]
+
+
+def _stream_objects():
+    out = []
+    for ct, tx, ty, w in STREAM_VARIANTS:
+        name = "%s_%dx%d_w%d" % ("f32" if ct == "float" else "f64", tx, ty, w)
+        out.append(("dt_grad_stream.hip", "-ffp-contract=fast",
+                    ["-DDT_STREAM_CT=%s" % ct, "-DDT_STREAM_TX=%d" % tx, "-DDT_STREAM_TY=%d" % ty,
+                     "-DDT_STREAM_MINW=%d" % w, "-DDT_STREAM_NAME=%s" % name], "dt_grad_stream_%s.o" % name))
+    return out
 
 
 def _run(cmd, verbose):
@@ -49,13 +65,27 @@ def _run(cmd, verbose):
     subprocess.run(cmd, check=True)
 
 
-def build_library(verbose=True):
-    objs = []
-    for src, contract in HIP_SOURCES:
-        obj = os.path.join(CSRC, src + ".o")
+def build_library(verbose=True, jobs=None):
+    units = [(src, contract, [], src + ".o") for src, contract in HIP_SOURCES] + _stream_objects()
+    cmds, objs = [], []
+    for src, contract, defines, oname in units:
+        obj = os.path.join(CSRC, oname)
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        _run([HIPCC] + HIP_FLAGS + [contract] + lang + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        cmds.append([HIPCC] + HIP_FLAGS + [contract] + defines + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
+    running = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        running.append(subprocess.Popen(cmd))
+        if len(running) >= jobs:
+            p = running.pop(0)
+            if p.wait() != 0:
+                raise subprocess.CalledProcessError(p.returncode, p.args)
+    for p in running:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, p.args)
     tmp = LIB_PATH + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)

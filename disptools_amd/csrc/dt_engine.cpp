@@ -124,6 +124,8 @@ struct dt_engine {
     int tz_policy = 0;
     int max_slots = 256;
     hipStream_t st = nullptr, io = nullptr, cap = nullptr;
+    hipStream_t side = nullptr, cap_side = nullptr;   // greedy kernels, forked from / joined to st (cap)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool own_st = false;
 
     char *pool_mem = nullptr;
@@ -244,15 +246,27 @@ int rebuild_tables(dt_engine *e, int stage)
     return 0;
 }
 
-int launch_passes(dt_engine *e, hipStream_t s)
+// One chunk of passes: the gradient passes on `s`, the greedy kernels on
+// `s2` (forked from `s` and joined back), so the two groups overlap.
+int launch_passes(dt_engine *e, hipStream_t s, hipStream_t s2)
 {
+    const bool both = e->ntiles[0] > 0 && e->ntiles[1] > 0;
+    hipStream_t sg = both ? s2 : s;
+    if (both) {
+        DT_HIP(hipEventRecord(e->ev_fork, s));
+        DT_HIP(hipStreamWaitEvent(s2, e->ev_fork, 0));
+    }
     for (int i = 0; i < e->chunk; ++i) {
         if (e->ntiles[0] > 0)
             DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, e->d_tiles[0], e->ntiles[0], s));
         if (e->ntiles[1] > 0) {
-            DT_HIP(launch_greedy_step(e->d_insts, e->d_tiles[1], e->ntiles[1], s));
-            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, e->d_tiles[1], e->ntiles[1], s));
+            DT_HIP(launch_greedy_step(e->d_insts, e->d_tiles[1], e->ntiles[1], sg));
+            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, e->d_tiles[1], e->ntiles[1], sg));
         }
+    }
+    if (both) {
+        DT_HIP(hipEventRecord(e->ev_join, s2));
+        DT_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
     }
     return 0;
 }
@@ -265,7 +279,7 @@ int build_graph(dt_engine *e)
     }
     hipGraph_t g = nullptr;
     DT_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = launch_passes(e, e->cap);
+    const int rc = launch_passes(e, e->cap, e->cap_side);
     const hipError_t ce = hipStreamEndCapture(e->cap, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -294,7 +308,7 @@ int enqueue_chunk(dt_engine *e)
         }
         DT_HIP(hipGraphLaunch(e->graph, e->st));
     } else {
-        if (launch_passes(e, e->st)) return -1;
+        if (launch_passes(e, e->st, e->side)) return -1;
     }
     int hi = 0;
     for (int s = 0; s < (int)e->slots.size(); ++s) {
@@ -437,6 +451,7 @@ void destroy(dt_engine *e)
     (void)hipSetDevice(e->device);
     if (e->st) (void)hipStreamSynchronize(e->st);
     if (e->io) (void)hipStreamSynchronize(e->io);
+    if (e->side) (void)hipStreamSynchronize(e->side);
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
     for (auto &c : e->inflight) (void)hipEventDestroy(c.ev);
     for (auto ev : e->chunk_events) (void)hipEventDestroy(ev);
@@ -460,6 +475,10 @@ void destroy(dt_engine *e)
     if (e->d_counter) (void)hipFree(e->d_counter);
     if (e->pool_mem) (void)hipFree(e->pool_mem);
     if (e->cap) (void)hipStreamDestroy(e->cap);
+    if (e->cap_side) (void)hipStreamDestroy(e->cap_side);
+    if (e->side) (void)hipStreamDestroy(e->side);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->io) (void)hipStreamDestroy(e->io);
     if (e->own_st && e->st) (void)hipStreamDestroy(e->st);
     delete e;
@@ -492,6 +511,10 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     }
     DT_HIP(hipStreamCreateWithFlags(&e->io, hipStreamNonBlocking));
     DT_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
+    DT_HIP(hipStreamCreateWithFlags(&e->cap_side, hipStreamNonBlocking));
+    DT_HIP(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    DT_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    DT_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
 
     if (hipMalloc(&e->pool_mem, budget) != hipSuccess) {
         (void)hipGetLastError();

@@ -432,6 +432,43 @@ static int occupancy_ct(int variant)
     }
 }
 
+// Streaming variants, one object per (precision, tile, register cap)
+// (dt_grad_stream.hip); variant STREAM_BASE + k is entry k of this list.
+#define DT_STREAM_VARIANTS(X) X(0, 32x16_w3) X(1, 32x20_w3) X(2, 64x8_w3) X(3, 32x24_w3) X(4, 64x12_w3)
+#define DT_DECLARE_STREAM(k, name)                                                                          \
+    hipError_t stream_launch_f32_##name(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st); \
+    hipError_t stream_launch_f64_##name(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st); \
+    int stream_occupancy_f32_##name();                                                                      \
+    int stream_occupancy_f64_##name();
+DT_STREAM_VARIANTS(DT_DECLARE_STREAM)
+
+hipError_t launch_stream_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
+                              hipStream_t st)
+{
+    const bool d = precision == 64;
+#define DT_LAUNCH_CASE(k, name)                                                                 \
+    case k:                                                                                     \
+        return d ? stream_launch_f64_##name(insts, tiles, nblocks, st)                          \
+                 : stream_launch_f32_##name(insts, tiles, nblocks, st);
+    switch (variant - STREAM_BASE) {
+        DT_STREAM_VARIANTS(DT_LAUNCH_CASE)
+    default: return hipErrorInvalidValue;
+    }
+#undef DT_LAUNCH_CASE
+}
+
+int stream_pass_blocks_per_cu(int precision, int variant)
+{
+    const bool d = precision == 64;
+#define DT_OCC_CASE(k, name) \
+    case k: return d ? stream_occupancy_f64_##name() : stream_occupancy_f32_##name();
+    switch (variant - STREAM_BASE) {
+        DT_STREAM_VARIANTS(DT_OCC_CASE)
+    default: return 1;
+    }
+#undef DT_OCC_CASE
+}
+
 hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                             hipStream_t st)
 {

@@ -4,19 +4,21 @@
 // weighted cofactors on plane zz; adjoint-stencil gradient and speculative
 // candidate on plane zz-1; FP64 partial sums; controller in the last block),
 // src/displacement_field_gradient.c:26-175 and src/jacobian.c:70-242, with a
-// memory schedule built for occupancy:
+// work layout balanced over the four SIMDs of a CU:
 //
+//  * every thread has one primary point: the first TX*TY threads own the
+//    tile's columns, the next 2(TX+TY) threads (in waves of their own) the
+//    ring around the tile, where W_{a,x} / W_{a,y} are needed across the tile
+//    edge.  Each wave therefore evaluates the Jacobian and cofactors once per
+//    plane; with 10 tile waves and 2 ring waves (32 x 20) every SIMD carries
+//    three of them;
+//  * a thread's first three load slots are its primary point's components,
+//    so the z-neighbours of its point come from registers;
 //  * LDS holds two trial planes (halo 3) and two planes of W_{a,x} / W_{a,y}
-//    (halo 1 along their own axis only), indexed by plane parity, so one
-//    barrier per plane separates every write from its reads;
-//  * each thread's first three load slots are its own column's components:
-//    the z-neighbours of its voxel come from registers, never from LDS;
-//  * the ring points of the tile (for W across the tile edge) belong to the
-//    first 2(TX+TY) threads, which load their ring point themselves and keep
-//    its z-column in a private LDS ring;
-//  * the remaining halo (rings 2 and 3, ring-1 corners) is spread over the
-//    other threads.
-// For a 32 x 16 tile this is 52.9 KB of LDS (three blocks of 8 waves per CU).
+//    (halo 1 along their own axis), indexed by plane parity: one barrier per
+//    plane separates every write from its reads;
+//  * the rest of the halo (rings 2 and 3, ring-1 corners) is spread over all
+//    threads.
 #include "dt_device.hpp"
 #include "dt_kernels.hpp"
 #include "dt_pass_common.hpp"
@@ -25,14 +27,14 @@ namespace dt {
 
 template <int TX, int TY>
 struct StreamGeom {
-    static constexpr int NT = TX * TY;
-    static constexpr int HX = TX + 6, HY = TY + 6, PL = HX * HY;
+    static constexpr int NOWN = TX * TY;                          // tile threads
     static constexpr int NRING = 2 * TX + 2 * TY;                 // ring-1 points without corners
-    static constexpr int NOTHER = PL - NT - NRING;                // rings 2-3 and ring-1 corners
-    static constexpr int PO = (3 * NOTHER + (NT - NRING) - 1) / (NT - NRING);
-    static constexpr int SX = PO > 3 ? PO : 3;                    // extra slots per thread
+    static constexpr int NT = NOWN + (NRING + 63) / 64 * 64;      // block size
+    static constexpr int HX = TX + 6, HY = TY + 6, PL = HX * HY;
+    static constexpr int NOTHER = PL - NOWN - NRING;              // rings 2-3 and ring-1 corners
+    static constexpr int SX = (3 * NOTHER + NT - 1) / NT;         // halo slots per thread
     static constexpr int SLOTS = 3 + SX;
-    static_assert(NRING <= NT, "one ring point per thread at most");
+    static_assert(NOWN % 64 == 0, "tile threads fill whole waves");
     static_assert(4 * HX + 4 * (HY - 4) + 4 == NOTHER, "halo decomposition");
 };
 
@@ -42,12 +44,11 @@ struct StreamSmem {
     float xs[2][3][G::PL];            // trial planes by parity, halo 3
     float wx[2][3][TY][TX + 2];       // W_{a,x} by parity, x-halo 1
     float wy[2][3][TY + 2][TX];       // W_{a,y} by parity, y-halo 1
-    float hc[5][3][G::NRING];         // ring points' z-columns (each entry private to its thread)
     double red[3 * (G::NT / 64)];
     int flag;
 };
 
-// (xx, yy) in the halo-3 plane of the r-th "other" point.
+// (xx, yy) in the halo-3 plane of the r-th point of rings 2-3 and ring-1 corners.
 template <int TX, int TY>
 __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 {
@@ -70,8 +71,8 @@ __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
     }
 }
 
-// Stage A at one point from its samples: J - 1, error, weight, weighted
-// cofactors.  what: 0 own point (all W), 1 x-ring (W_{a,x}), 2 y-ring (W_{a,y}).
+// Stage A at one point: J - 1, error, weight and the nine weighted cofactors
+// W[3 r + a] = w * M_{a, r} (r: relation axis x, y, z).
 template <typename CT>
 __device__ __forceinline__ void stream_stage_a(const PassParams &P, int zz, int gx, int gy,
                                                const float (&sm)[3][3][4], const float (&v0)[3], float T,
@@ -110,8 +111,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                                                  const TileCoord tc, double &Esum, float &mx, double &Gsum)
 {
     using G = StreamGeom<TX, TY>;
-    constexpr int NT = G::NT, HX = G::HX, PL = G::PL, NRING = G::NRING, NOTHER = G::NOTHER;
-    constexpr int SX = G::SX;
+    constexpr int NT = G::NT, NOWN = G::NOWN, HX = G::HX, PL = G::PL, NRING = G::NRING;
+    constexpr int NOTHER = G::NOTHER, SX = G::SX, SLOTS = G::SLOTS;
 
     const PassParams P = load_params(I);
     const int nx = P.nx, ny = P.ny, nz = P.nz;
@@ -133,69 +134,58 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const int z0 = tc.z0, z1 = tc.z1;
     const unsigned cn4 = 4u * n;   // bytes between components
 
+    // primary point of this thread: a tile column (tid < NOWN) or a ring point
     const int tid = threadIdx.x;
-    const int lx = tid % TX, ly = tid / TX;
-    const int gx = tc.x0 + lx, gy = tc.y0 + ly;
-    const bool col_in = gx < nx && gy < ny;
-    const bool col_interior = gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
-    const unsigned own_v = col_in ? 4u * (unsigned)(gy * nx + gx) : 0u;
-    const int c0i = (ly + 3) * HX + (lx + 3);
-
-    // ring point of this thread (threads 0 .. NRING-1)
-    const bool ring = tid < NRING;
-    int hx = 0, hy = 0, hwhat = 0;
-    if (ring) {
-        const int h = tid;
-        if (h < TY) { hx = -1; hy = h; hwhat = 1; }
-        else if (h < 2 * TY) { hx = TX; hy = h - TY; hwhat = 1; }
-        else if (h < 2 * TY + TX) { hx = h - 2 * TY; hy = -1; hwhat = 2; }
-        else { hx = h - 2 * TY - TX; hy = TY; hwhat = 2; }
+    const bool own = tid < NOWN;
+    const int rr = tid - NOWN;
+    const bool ring = !own && rr < NRING;
+    int lx = 0, ly = 0, what = 0;   // what: 0 tile, 1 x-ring (W_{a,x}), 2 y-ring (W_{a,y}), 3 none
+    if (own) {
+        lx = tid % TX; ly = tid / TX;
+    } else if (ring) {
+        if (rr < TY) { lx = -1; ly = rr; what = 1; }
+        else if (rr < 2 * TY) { lx = TX; ly = rr - TY; what = 1; }
+        else if (rr < 2 * TY + TX) { lx = rr - 2 * TY; ly = -1; what = 2; }
+        else { lx = rr - 2 * TY - TX; ly = TY; what = 2; }
+    } else {
+        what = 3;
     }
-    const int qx = tc.x0 + hx, qy = tc.y0 + hy;
-    const bool halo_in = ring && qx >= 0 && qx < nx && qy >= 0 && qy < ny;
-    const unsigned ring_v = halo_in ? 4u * (unsigned)(qy * nx + qx) : 0u;
-    const int h0i = (hy + 3) * HX + (hx + 3);
+    const int gx = tc.x0 + lx, gy = tc.y0 + ly;
+    const bool pin = what != 3 && gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
+    const bool col_interior = own && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
+    const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
+    const int pi = what == 3 ? -1 : (ly + 3) * HX + (lx + 3);
 
-    // extra load slots: ring threads load their ring point's components, the
-    // others a share of the outer halo
+    // halo slots: a share of rings 2-3 and the ring-1 corners
     unsigned xv[SX];   // voffset
     int xd[SX];        // LDS index in a plane triple, -1: none
     unsigned xok = 0;
 #pragma unroll
     for (int j = 0; j < SX; ++j) {
-        if (ring) {
-            xv[j] = j < 3 ? ring_v + (unsigned)j * cn4 : 0u;
-            xd[j] = j < 3 ? j * PL + h0i : -1;
-            xok |= (j < 3 && halo_in ? 1u : 0u) << j;
-        } else {
-            const int o = (tid - NRING) + j * (NT - NRING);
-            bool v = o < 3 * NOTHER;
-            int c = 0, xx = 0, yy = 0;
-            if (v) {
-                c = o / NOTHER;
-                other_point<TX, TY>(o - c * NOTHER, xx, yy);
-            }
-            const int x = tc.x0 - 3 + xx, y = tc.y0 - 3 + yy;
-            const bool in = v && x >= 0 && x < nx && y >= 0 && y < ny;
-            xv[j] = in ? 4u * ((unsigned)c * n + (unsigned)(y * nx + x)) : 0u;
-            xd[j] = v ? c * PL + yy * HX + xx : -1;
-            xok |= (in ? 1u : 0u) << j;
+        const int o = tid + j * NT;
+        const bool v = o < 3 * NOTHER;
+        int c = 0, xx = 0, yy = 0;
+        if (v) {
+            c = o / NOTHER;
+            other_point<TX, TY>(o - c * NOTHER, xx, yy);
         }
+        const int x = tc.x0 - 3 + xx, y = tc.y0 - 3 + yy;
+        const bool in = v && x >= 0 && x < nx && y >= 0 && y < ny;
+        xv[j] = in ? 4u * ((unsigned)c * n + (unsigned)(y * nx + x)) : 0u;
+        xd[j] = v ? c * PL + yy * HX + xx : -1;
+        xok |= (in ? 1u : 0u) << j;
     }
 
-    // raw loads of one plane: own column (3) and extra slots (SX); INTERP
-    // passes load base and candidate
-    struct Raw { float a[3 + SX]; float b[INTERP ? 3 + SX : 1]; };
-    auto plane_soff = [&](int gz) -> unsigned {
-        const int gzc = min(max(gz, 0), nz - 1);
-        return 4u * (unsigned)gzc * plane;
-    };
+    // raw loads of one plane: the primary point's components (3) and the
+    // halo slots; INTERP passes load base and candidate
+    struct Raw { float a[SLOTS]; float b[INTERP ? SLOTS : 1]; };
     auto fetch = [&](int gz, Raw &r) {
-        const unsigned so = plane_soff(gz);
+        const int gzc = min(max(gz, 0), nz - 1);
+        const unsigned so = 4u * (unsigned)gzc * plane;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r.a[c] = bload(r_src, own_v, so + (unsigned)c * cn4);
-            if (INTERP) r.b[c] = bload(r_cnd, own_v, so + (unsigned)c * cn4);
+            r.a[c] = bload(r_src, pv, so + (unsigned)c * cn4);
+            if (INTERP) r.b[c] = bload(r_cnd, pv, so + (unsigned)c * cn4);
         }
 #pragma unroll
         for (int j = 0; j < SX; ++j) {
@@ -204,42 +194,35 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         }
     };
     // trial values of a plane (zero outside the volume)
-    struct Val { float v[3 + SX]; };
+    struct Val { float v[SLOTS]; };
     auto resolve = [&](int gz, const Raw &r, Val &o) {
         const bool zin = gz >= 0 && gz < nz;
 #pragma unroll
-        for (int k = 0; k < 3 + SX; ++k) {
+        for (int k = 0; k < SLOTS; ++k) {
             float v = r.a[k];
             if (INTERP) v = v + s * (r.b[k] - v);
-            const bool ok = zin && (k < 3 ? col_in : ((xok >> (k - 3)) & 1u) != 0);
+            const bool ok = zin && (k < 3 ? pin : ((xok >> (k - 3)) & 1u) != 0);
             o.v[k] = ok ? v : 0.0f;
         }
     };
     auto commit = [&](int gz, const Val &o) {
         float *dst = &S.xs[gz & 1][0][0];
+        if (pi >= 0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) dst[c * PL + c0i] = o.v[c];
+            for (int c = 0; c < 3; ++c) dst[c * PL + pi] = o.v[c];
+        }
 #pragma unroll
         for (int j = 0; j < SX; ++j)
             if (xd[j] >= 0) dst[xd[j]] = o.v[3 + j];
     };
-    auto ring_col = [&](int gz, const Val &o) {
-        if (ring) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) S.hc[(gz + 10) % 5][c][tid] = o.v[3 + c];
-        }
-    };
-    struct TM { float ti, th; unsigned mi, mh; };
+    struct TM { float t; unsigned m; };
     auto fetch_tm = [&](int gz, TM &o) {
         const int gzc = min(max(gz, 0), nz - 1);
-        const unsigned so4 = 4u * (unsigned)gzc * plane, so1 = (unsigned)gzc * plane;
-        o.ti = bload(r_tgt, own_v, so4);
-        o.th = bload(r_tgt, ring_v, so4);
-        o.mi = (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r_msk, (int)(own_v >> 2), (int)so1, 0);
-        o.mh = (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r_msk, (int)(ring_v >> 2), (int)so1, 0);
+        o.t = bload(r_tgt, pv, 4u * (unsigned)gzc * plane);
+        o.m = (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r_msk, (int)(pv >> 2), (int)((unsigned)gzc * plane), 0);
     };
 
-    // own column window: planes zz-2 .. zz+2 at stage A of plane zz
+    // z-window of the primary point: planes zz-2 .. zz+2 at stage A of plane zz
     float col[5][3];
     Raw rn;          // raw loads of plane zz+2 in flight at step zz
     Val vn;          // resolved plane zz+1 (committed at step zz)
@@ -254,20 +237,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         Val v;
         resolve(z0 - 3, r0, v);
         col[1][0] = v.v[0]; col[1][1] = v.v[1]; col[1][2] = v.v[2];
-        ring_col(z0 - 3, v);
         fetch(z0 - 1, r0);
         resolve(z0 - 2, r1, v);
         col[2][0] = v.v[0]; col[2][1] = v.v[1]; col[2][2] = v.v[2];
-        ring_col(z0 - 2, v);
         fetch(z0, r1);
         resolve(z0 - 1, r0, v);
         col[3][0] = v.v[0]; col[3][1] = v.v[1]; col[3][2] = v.v[2];
-        ring_col(z0 - 1, v);
         commit(z0 - 1, v);
         fetch(z0 + 1, rn);
         resolve(z0, r1, vn);
         col[4][0] = vn.v[0]; col[4][1] = vn.v[1]; col[4][2] = vn.v[2];
-        ring_col(z0, vn);
         col[0][0] = col[0][1] = col[0][2] = 0.0f;
     }
     __syncthreads();
@@ -285,64 +264,42 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             col[0][c] = col[1][c]; col[1][c] = col[2][c]; col[2][c] = col[3][c]; col[3][c] = col[4][c];
             col[4][c] = vz1.v[c];
         }
-        ring_col(zz + 2, vz1);
         fetch(zz + 3, rn);
         const TM cur = tm;
         fetch_tm(zz + 1, tm);
 
-        // (A) plane zz
+        // (A) plane zz at the primary point
         CT wz_p[3] = {0, 0, 0};
-        if (zz >= 0 && zz < nz) {
-            const bool count = zz >= z0 && zz < z1;
-            const float *Q = &S.xs[zz & 1][0][0];
-            if (col_in) {
-                float sm[3][3][4];
+        if (zz >= 0 && zz < nz && pin) {
+            const float *R0 = &S.xs[zz & 1][0][0] + pi;
+            float sm[3][3][4];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float *R = Q + c * PL + c0i;
-                    sm[c][0][0] = R[-2]; sm[c][0][1] = R[-1]; sm[c][0][2] = R[1]; sm[c][0][3] = R[2];
-                    sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
-                    sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
-                }
-                CT W[9];
-                stream_stage_a<CT>(P, zz, gx, gy, sm, col[2], cur.ti, cur.mi != 0, count, Esum, mx, W);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    S.wx[zz & 1][a][ly][lx + 1] = (float)W[a];
-                    S.wy[zz & 1][a][ly + 1][lx] = (float)W[3 + a];
-                    wz_p[a] = (CT)(float)W[6 + a];
-                }
+            for (int c = 0; c < 3; ++c) {
+                const float *R = R0 + c * PL;
+                sm[c][0][0] = R[-2]; sm[c][0][1] = R[-1]; sm[c][0][2] = R[1]; sm[c][0][3] = R[2];
+                sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
+                sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
             }
-            if (halo_in) {
-                float sm[3][3][4], v0[3];
+            const bool count = own && zz >= z0 && zz < z1;
+            CT W[9];
+            stream_stage_a<CT>(P, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Esum, mx, W);
+            const int pz = zz & 1;
+            if (what != 2) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float *R = Q + c * PL + h0i;
-                    v0[c] = S.hc[(zz + 10) % 5][c][tid];
-                    sm[c][0][0] = R[-2]; sm[c][0][1] = R[-1]; sm[c][0][2] = R[1]; sm[c][0][3] = R[2];
-                    sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
-                    sm[c][2][0] = S.hc[(zz + 8) % 5][c][tid];
-                    sm[c][2][1] = S.hc[(zz + 9) % 5][c][tid];
-                    sm[c][2][2] = S.hc[(zz + 11) % 5][c][tid];
-                    sm[c][2][3] = S.hc[(zz + 12) % 5][c][tid];
-                }
-                CT W[9];
-                double e0 = 0.0;
-                float m0 = 0.0f;
-                stream_stage_a<CT>(P, zz, qx, qy, sm, v0, cur.th, cur.mh != 0, false, e0, m0, W);
-                if (hwhat == 1) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) S.wx[zz & 1][a][hy][hx + 1] = (float)W[a];
-                } else {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) S.wy[zz & 1][a][hy + 1][hx] = (float)W[3 + a];
-                }
+                for (int a = 0; a < 3; ++a) S.wx[pz][a][ly][lx + 1] = (float)W[a];
             }
+            if (what != 1) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) S.wy[pz][a][ly + 1][lx] = (float)W[3 + a];
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
         }
 
-        // (B) plane zb = zz-1: W_{a,x}, W_{a,y} of plane zb (written last step)
+        // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
+        // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
-        if (zb >= z0 && zb < z1 && col_in) {
+        if (own && pin && zb >= z0 && zb < z1) {
             const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
             if (col_interior && zb >= 1 && zb <= nz - 2) {
@@ -361,14 +318,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             const float ux = col[1][0], uy = col[1][1], uz = col[1][2];
             const unsigned so = 4u * (unsigned)zb * plane;
             if (has_next) {
-                bstore(r_next, ux - eta_next * f0, own_v, so);
-                bstore(r_next, uy - eta_next * f1, own_v, so + cn4);
-                bstore(r_next, uz - eta_next * f2, own_v, so + 2u * cn4);
+                bstore(r_next, ux - eta_next * f0, pv, so);
+                bstore(r_next, uy - eta_next * f1, pv, so + cn4);
+                bstore(r_next, uz - eta_next * f2, pv, so + 2u * cn4);
             }
             if (has_trial) {
-                bstore(r_trial, ux, own_v, so);
-                bstore(r_trial, uy, own_v, so + cn4);
-                bstore(r_trial, uz, own_v, so + 2u * cn4);
+                bstore(r_trial, ux, pv, so);
+                bstore(r_trial, uy, pv, so + cn4);
+                bstore(r_trial, uz, pv, so + 2u * cn4);
             }
         }
 #pragma unroll
@@ -380,17 +337,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         __syncthreads();
     };
 
-    for (int zz = z0 - 1; zz <= z1; zz += 2) {
-        step(zz);
-        if (zz + 1 <= z1) step(zz + 1);
-    }
+    // not unrolled: two steps in flight at once would double the live registers
+#pragma unroll 1
+    for (int zz = z0 - 1; zz <= z1; ++zz) step(zz);
 }
 
 template <typename CT, int TX, int TY, int MINW>
-__global__ void __launch_bounds__(TX * TY, MINW)
+__global__ void __launch_bounds__((StreamGeom<TX, TY>::NT), MINW)
 k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
 {
-    constexpr int NT = TX * TY;
+    constexpr int NT = StreamGeom<TX, TY>::NT;
     __shared__ StreamSmem<TX, TY> S;
 
     const Tile tl = tiles[blockIdx.x];
@@ -420,56 +376,36 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
     }
 }
 
-template <typename CT, int TX, int TY, int MINW>
-static hipError_t stream_launch(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st)
+// One (precision, tile) variant per compilation of this file: the register
+// allocation of a kernel changes when other instantiations share its
+// translation unit, so each variant is built alone (DT_STREAM_CT, DT_STREAM_TX,
+// DT_STREAM_TY, DT_STREAM_MINW -- waves per SIMD the registers are capped
+// for -- and DT_STREAM_NAME set by the build).
+#if defined(DT_STREAM_CT) && defined(DT_STREAM_TX) && defined(DT_STREAM_TY) && defined(DT_STREAM_NAME) && \
+    defined(DT_STREAM_MINW)
+#define DT_PASTE2(a, b) a##b
+#define DT_PASTE(a, b) DT_PASTE2(a, b)
+
+hipError_t DT_PASTE(stream_launch_, DT_STREAM_NAME)(const Inst *insts, const Tile *tiles, int nblocks,
+                                                     hipStream_t st)
 {
-    hipLaunchKernelGGL((k_grad_stream<CT, TX, TY, MINW>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles);
+    constexpr int NT = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NT;
+    hipLaunchKernelGGL((k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>), dim3(nblocks),
+                       dim3(NT), 0, st, insts, tiles);
     return hipGetLastError();
 }
 
-template <typename CT, int TX, int TY, int MINW>
-static int stream_occupancy()
+int DT_PASTE(stream_occupancy_, DT_STREAM_NAME)()
 {
+    constexpr int NT = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NT;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_grad_stream<CT, TX, TY, MINW>, TX * TY, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>, NT, 0) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }
-
-// Variants STREAM_BASE + k: see grad_variant_dims.
-template <typename CT>
-static hipError_t stream_launch_ct(int v, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st)
-{
-    switch (v - STREAM_BASE) {
-    case 1: return stream_launch<CT, 32, 8, 5>(insts, tiles, nblocks, st);
-    case 2: return stream_launch<CT, 64, 8, 6>(insts, tiles, nblocks, st);
-    case 3: return stream_launch<CT, 32, 16, 4>(insts, tiles, nblocks, st);
-    default: return stream_launch<CT, 32, 16, 6>(insts, tiles, nblocks, st);
-    }
-}
-
-template <typename CT>
-static int stream_occupancy_ct(int v)
-{
-    switch (v - STREAM_BASE) {
-    case 1: return stream_occupancy<CT, 32, 8, 5>();
-    case 2: return stream_occupancy<CT, 64, 8, 6>();
-    case 3: return stream_occupancy<CT, 32, 16, 4>();
-    default: return stream_occupancy<CT, 32, 16, 6>();
-    }
-}
-
-hipError_t launch_stream_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
-                              hipStream_t st)
-{
-    return precision == 64 ? stream_launch_ct<double>(variant, insts, tiles, nblocks, st)
-                           : stream_launch_ct<float>(variant, insts, tiles, nblocks, st);
-}
-
-int stream_pass_blocks_per_cu(int precision, int variant)
-{
-    return precision == 64 ? stream_occupancy_ct<double>(variant) : stream_occupancy_ct<float>(variant);
-}
+#else
+#error "dt_grad_stream.hip is compiled once per variant (see disptools_amd/build.py)"
+#endif
 
 }  // namespace dt

@@ -16,19 +16,22 @@ hipError_t launch_jacobian(const float *f, float *J, int nx, int ny, int nz, flo
 hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st);
 // Variants of the fused gradient pass: column tile TX x TY (one thread per
 // column) and the waves per SIMD its registers are capped for (3: may spill a
-// few registers; 2: no spill, lower occupancy; 4: spills more).
+// few registers; 2: no spill, lower occupancy; 4: spills more).  Variants
+// 0-7 are the two-barrier form (dt_grad_pass.hip) kept for comparison.
 // Variants SPLIT_BASE .. SPLIT_BASE+2 run each pass as two kernels
 // (dt_split_pass.hip); variants from STREAM_BASE on are the single-barrier
 // streaming form (dt_grad_stream.hip).
-constexpr int GRAD_VARIANTS = 15;
+constexpr int GRAD_VARIANTS = 16;
 constexpr int SPLIT_BASE = 8;
 constexpr int STREAM_BASE = 11;
 inline void grad_variant_dims(int v, int &tx, int &ty)
 {
     switch (v) {
-    case STREAM_BASE: case STREAM_BASE + 3: tx = 32; ty = 16; break;
-    case STREAM_BASE + 1: tx = 32; ty = 8; break;
+    case STREAM_BASE: tx = 32; ty = 16; break;
+    case STREAM_BASE + 1: tx = 32; ty = 20; break;
     case STREAM_BASE + 2: tx = 64; ty = 8; break;
+    case STREAM_BASE + 3: tx = 32; ty = 24; break;
+    case STREAM_BASE + 4: tx = 64; ty = 12; break;
     case SPLIT_BASE: tx = 32; ty = 16; break;
     case SPLIT_BASE + 1: tx = 32; ty = 8; break;
     case SPLIT_BASE + 2: tx = 64; ty = 8; break;

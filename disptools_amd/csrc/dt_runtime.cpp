@@ -15,7 +15,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "disptools_hip.h"
@@ -207,9 +209,10 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
     return best;
 }
 
-// Default pass variant: the fused pass on 32 x 16 column tiles (measured fastest
-// at 128^3 among the compiled variants); DISPTOOLS_GRAD_VARIANT overrides it.
-constexpr int DEFAULT_GRAD_VARIANT = 1;
+// Default pass variant: the single-barrier streaming pass on 32 x 16 column
+// tiles with the tile ring in waves of its own (measured fastest at 128^3
+// among the compiled variants, with 64 x 12); DISPTOOLS_GRAD_VARIANT overrides it.
+constexpr int DEFAULT_GRAD_VARIANT = STREAM_BASE;
 
 int variant_from_env()
 {
@@ -218,6 +221,20 @@ int variant_from_env()
     const int v = std::atoi(s);
     return (v >= 0 && v < GRAD_VARIANTS) ? v : DEFAULT_GRAD_VARIANT;
 }
+
+// Variants the autotuner times: the fused passes (the two-kernel variants
+// need a weight map and are slower everywhere measured).
+static const int TUNE_CANDIDATES[] = {1, STREAM_BASE, STREAM_BASE + 1, STREAM_BASE + 2, STREAM_BASE + 3,
+                                      STREAM_BASE + 4};
+
+struct TuneKey {
+    int dev, nx, ny, nz;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(dev, nx, ny, nz) < std::tie(o.dev, o.nx, o.ny, o.nz);
+    }
+};
+static std::map<TuneKey, int> g_tuned;
 
 int precision_from_env()
 {
@@ -280,6 +297,139 @@ int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy,
     return 0;
 }
 
+// Times the fused-pass variants on a synthetic instance of this shape: a
+// ball of J = 1.05 (mask = the ball) in J = 1, zero initial field, the solve
+// control running with its stop tests disabled (epsilon = 0, theta = -1,
+// iota = 0), so every timed launch evaluates a trial.
+static int tune_gradient(int nx, int ny, int nz, float dx, float dy, float dz, int passes, dt_tune_result *out)
+{
+    if (passes <= 0) passes = 8;
+    const int warm = 2;
+    const long long n = (long long)nx * ny * nz;
+    const int precision = precision_from_env();
+    std::vector<float> T((size_t)n);
+    std::vector<uint8_t> M((size_t)n);
+    const double cx = 0.5 * (nx - 1), cy = 0.5 * (ny - 1), cz = 0.5 * (nz - 1);
+    const double r = std::max(1.0, std::min(nx, std::min(ny, nz)) / 3.0);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t i = ((size_t)z * ny + y) * nx + x;
+                const bool in = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz) <= r * r;
+                T[i] = in ? 1.05f : 1.0f;
+                M[i] = in ? 1 : 0;
+            }
+    dt_params p{};
+    p.epsilon = 0.0f; p.tolerance = 0.2f; p.eta = 0.1f; p.eta_max = 0.4f; p.alpha = 1.2f; p.beta = 0.5f;
+    p.gamma = 0.1f; p.delta = 1e-3f; p.zeta = 10.0f; p.theta = -1.0f; p.iota = 0.0f; p.strict = 0;
+    p.it_max = 1LL << 40;
+
+    size_t need = align_up(12 * (size_t)n, 256) + align_up(4 * (size_t)n, 256) + align_up((size_t)n, 256);
+    size_t most = 0;
+    for (int v : TUNE_CANDIDATES) {
+        const Plan P = make_plan(DT_GRADIENT, nx, ny, nz, precision, v, 0);
+        most = std::max(most, scratch_bytes(P, 0, true) + align_up(sizeof(Inst), 256) +
+                                  align_up((size_t)P.ntiles * sizeof(Tile), 256));
+    }
+    Arena *A = nullptr;
+    if (arena_get(need + most + 1024, &A)) return -1;
+    Carver cv(A->ptr);
+    float *d_field = cv.take<float>(3 * (size_t)n);
+    float *d_T = cv.take<float>((size_t)n);
+    uint8_t *d_M = cv.take<uint8_t>((size_t)n);
+    const size_t base_off = cv.off;
+
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemcpyAsync(d_T, T.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_M, M.data(), (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error(__LINE__, __FILE__, __func__, "cannot stage the tuning volume");
+        rc = -1;
+    }
+    out->count = 0;
+    out->best_variant = DEFAULT_GRAD_VARIANT;
+    double best = 1e300;
+    for (int v : TUNE_CANDIDATES) {
+        if (rc) break;
+        const Plan P = make_plan(DT_GRADIENT, nx, ny, nz, precision, v, 0);
+        cv.off = base_off;
+        const Inst h = make_inst(P, dx, dy, dz, p, d_T, d_M, d_field, cv, nullptr, 0, nullptr, nullptr);
+        Inst *d_inst = cv.take<Inst>(1);
+        Tile *d_tiles = cv.take<Tile>((size_t)P.ntiles);
+        std::vector<Tile> tiles;
+        append_tiles_xcd(tiles, {{0, P.ntiles}});
+        const Ctrl c = initial_ctrl(DT_GRADIENT, p);
+        float ms = 0.0f;
+        if (hipMemsetAsync(d_field, 0, 12 * (size_t)n, st) != hipSuccess ||
+            hipMemcpyAsync(d_inst, &h, sizeof(Inst), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(h.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemsetAsync(h.counter, 0, sizeof(unsigned int), st) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot stage a tuning instance");
+            rc = -1;
+            break;
+        }
+        for (int i = 0; i < warm && !rc; ++i)
+            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, st) != hipSuccess) rc = -1;
+        if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = -1;
+        for (int i = 0; i < passes && !rc; ++i)
+            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, st) != hipSuccess) rc = -1;
+        if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = -1;
+        Ctrl fin{};
+        if (!rc && hipMemcpy(&fin, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+        if (rc) {
+            set_error(__LINE__, __FILE__, __func__, "tuning pass failed on the device");
+            break;
+        }
+        const double us = 1e3 * ms / passes;
+        if (out->count < DT_TUNE_MAX) {
+            out->variant[out->count] = v;
+            out->us_per_pass[out->count] = us;
+            out->count++;
+        }
+        // a solve that stopped during the timing ran idle passes: not comparable
+        if (fin.phase != PH_DONE && us < best) {
+            best = us;
+            out->best_variant = v;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (rc) return -1;
+    int dev = 0;
+    DT_HIP(hipGetDevice(&dev));
+    g_tuned[TuneKey{dev, nx, ny, nz}] = out->best_variant;
+    return 0;
+}
+
+// Gradient-pass variant of a solve: DISPTOOLS_GRAD_VARIANT, else the tuned
+// variant of the shape when DISPTOOLS_AUTOTUNE=1, else the default.
+static int solve_variant(int algorithm, int nx, int ny, int nz, float dx, float dy, float dz)
+{
+    if (algorithm != DT_GRADIENT) return 0;
+    if (std::getenv("DISPTOOLS_GRAD_VARIANT")) return variant_from_env();
+    const char *t = std::getenv("DISPTOOLS_AUTOTUNE");
+    if (!t || std::strcmp(t, "1") != 0) return DEFAULT_GRAD_VARIANT;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return DEFAULT_GRAD_VARIANT;
+    auto it = g_tuned.find(TuneKey{dev, nx, ny, nz});
+    if (it != g_tuned.end()) return it->second;
+    dt_tune_result r{};
+    if (tune_gradient(nx, ny, nz, dx, dy, dz, 0, &r)) {
+        disptools_clear_error();
+        return DEFAULT_GRAD_VARIANT;
+    }
+    return r.best_variant;
+}
+
 // Core solve on device-resident buffers.
 static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float dy, float dz,
                         const float *d_target, const uint8_t *d_mask, const dt_params *p, float *d_field,
@@ -287,7 +437,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
 {
     const auto t_start = std::chrono::steady_clock::now();
     const int precision = precision_from_env();
-    const Plan P = make_plan(algorithm, nx, ny, nz, precision, variant_from_env(), 0);
+    const Plan P = make_plan(algorithm, nx, ny, nz, precision, solve_variant(algorithm, nx, ny, nz, dx, dy, dz), 0);
     const long long n = P.n;
     const int variant = P.variant;
     const int ntiles = P.ntiles;
@@ -537,4 +687,15 @@ extern "C" void regularise(const size_t nx, const size_t ny, const size_t nz, fl
         set_error(__LINE__, __FILE__, __func__, "regularise failed on the device");
     }
     (void)hipFree(mem);
+}
+
+extern "C" int dt_tune_gradient(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, int passes,
+                                dt_tune_result *out)
+{
+    std::lock_guard<std::mutex> g(g_lock);
+    disptools_clear_error();
+    dt_params p{};
+    if (validate(DT_GRADIENT, nx, ny, nz, dx, dy, dz, &p)) return -1;
+    if (!out) DT_FAIL("null result");
+    return tune_gradient((int)nx, (int)ny, (int)nz, dx, dy, dz, passes, out);
 }

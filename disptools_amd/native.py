@@ -149,6 +149,16 @@ class EngineStats(ctypes.Structure):
 
 
 ENGINE_FULL = -2
+TUNE_MAX = 32
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best_variant", ctypes.c_int), ("count", ctypes.c_int),
+                ("variant", ctypes.c_int * TUNE_MAX), ("us_per_pass", ctypes.c_double * TUNE_MAX)]
+
+
+lib.dt_tune_gradient.argtypes = [_sz, _sz, _sz, _f, _f, _f, ctypes.c_int, ctypes.POINTER(TuneResult)]
+lib.dt_tune_gradient.restype = ctypes.c_int
 
 lib.dt_engine_create.argtypes = [ctypes.POINTER(EngineConfig), _vp]
 lib.dt_engine_create.restype = _vp
@@ -257,6 +267,15 @@ def solve_device(algorithm, nx, ny, nz, spacing, d_target, d_mask, d_field, stre
     if rc != 0:
         raise HipError(last_error())
     return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def tune_gradient(nx, ny, nz, spacing=(1.0, 1.0, 1.0), passes=0):
+    """Times the fused gradient-pass variants for this shape; returns
+    (best variant, {variant: microseconds per pass})."""
+    r = TuneResult()
+    if lib.dt_tune_gradient(nx, ny, nz, *map(float, spacing), int(passes), ctypes.byref(r)) != 0:
+        raise HipError(last_error())
+    return int(r.best_variant), {int(r.variant[i]): float(r.us_per_pass[i]) for i in range(r.count)}
 
 
 def jacobian_device(nx, ny, nz, spacing, d_field, d_J, stream=0):

@@ -144,6 +144,29 @@ int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
                   dt_solve_info *info, double *trace, long long trace_cap);
 
 /* ------------------------------------------------------------------------ */
+/* Autotuner of the gradient pass (no counterpart in the reference).        */
+/* ------------------------------------------------------------------------ */
+
+#define DT_TUNE_MAX 32
+
+typedef struct dt_tune_result {
+    int best_variant;                 /* fastest variant for the shape          */
+    int count;                        /* variants timed                         */
+    int variant[DT_TUNE_MAX];
+    double us_per_pass[DT_TUNE_MAX];  /* mean time of one pass (HIP events)     */
+} dt_tune_result;
+
+/* Times `passes` passes (0: 8) of every fused gradient-pass variant (tile
+ * shapes and register budgets) on a synthetic volume of this shape, inputs
+ * resident in HBM, and records the fastest for the shape on the current
+ * device.  With DISPTOOLS_AUTOTUNE=1 in the environment, dt_solve_* calls
+ * use the recorded variant for their shape (tuning it on first use);
+ * DISPTOOLS_GRAD_VARIANT overrides both.  The variants agree to the solver's
+ * parity bar, not bit for bit.  Returns 0 or -1 (error state set). */
+int dt_tune_gradient(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, int passes,
+                     dt_tune_result *out);
+
+/* ------------------------------------------------------------------------ */
 /* Batch engine: continuous batching of solves of uneven sizes.             */
 /* No counterpart in the reference (one solve per call there); for callers  */
 /* that serve many volumes: every pass launch covers all admitted volumes,  */

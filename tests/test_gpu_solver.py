@@ -196,3 +196,21 @@ def test_python_displacement_pads_and_crops(gpu_lib, oracle32, oracle64):
     fo, io = oracle64.solve("gradient", Tp.astype(np.float64), Mp, it_max=200)
     fo = np.moveaxis(fo, 0, -1)[1:-1, 1:-1, 1:-1]
     assert np.abs(out - fo).max() < 5e-3
+
+
+def test_autotuner_picks_a_timed_variant_and_keeps_parity(gpu_lib, oracle32, oracle64, monkeypatch):
+    best, times = gpu_lib.tune_gradient(40, 40, 40)
+    print("tuned 40^3:", best, {k: round(v, 1) for k, v in times.items()})
+    assert best in times and len(times) >= 4
+    assert all(t > 0 for t in times.values())
+    assert times[best] == min(times.values())
+    # a tuned solve of the same shape keeps the parity bar
+    T, M, sp, it_max = CASES["config1_40"]()
+    monkeypatch.setenv("DISPTOOLS_AUTOTUNE", "1")
+    monkeypatch.delenv("DISPTOOLS_GRAD_VARIANT", raising=False)
+    fg, ig, _ = gpu_lib.solve_host("gradient", T, M, sp, it_max=it_max)
+    fo, io = oracle64.solve("gradient", T.astype(np.float64), M, sp, it_max=it_max)
+    assert ig["reason"] == io["reason"]
+    _, mo = masked_max_error(oracle32, fo.astype(np.float32), T, M, sp)
+    _, mg = masked_max_error(oracle32, fg, T, M, sp)
+    assert abs(mg - mo) <= MAX_ERR_TOL
