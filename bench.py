@@ -8,7 +8,7 @@ each step resets the field to zero and solves it to termination.
 
 value = sum over steps of (voxels x iterations) / wall time of the steps.
 
-roofline: the fused gradient pass (k_grad_pass).  Algorithmic bytes per
+roofline: the fused gradient pass (k_grad_stream).  Algorithmic bytes per
 evaluated pass = 12 B per field volume read or written (the controller counts
 them: 1 read + 1 speculative write on an accepted-path pass, 2 reads + 2
 writes after a rejection) + 5 B of target and mask, per voxel; achieved =
@@ -34,11 +34,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel-iters/s at 128³ + p50/p99 solve latency @ fixed QPS vs plain GPU solve"
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per k_grad_pass launch of the default solve (128^3, it_max 2000,
+# HBM bytes per k_grad_stream launch of the default solve (128^3, it_max 2000,
 # default pass variant), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # runs of this script, scaled by what the counters read for a 1 GiB copy
 # (--mode copy): read bytes = 2.0 x FETCH_SIZE, written = 1.0 x WRITE_SIZE.
-MEASURED_TRAFFIC_128 = 68.85e6
+MEASURED_TRAFFIC_128 = 68.00e6
 
 
 def parse():
@@ -411,7 +411,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": MEASURED_TRAFFIC_128 if default_workload else None,
-                     "kernel": "k_grad_pass", "avg_launch_us": avg_launch_s * 1e6,
+                     "kernel": "k_grad_stream", "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes / launches},
         "cpu_baseline": cpu,
         "batch": batch,

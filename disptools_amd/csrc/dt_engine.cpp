@@ -2,16 +2,19 @@
 //
 // Every admitted instance (a volume with its parameters) owns a slot: an
 // Inst record and a control block in slot-indexed device arrays, and a range
-// of a device pool holding its solver state.  A pass is one launch of the
-// fused gradient kernel over the tile table of all gradient instances, plus
-// the greedy step and evaluation over the table of all greedy instances;
-// each instance's last block runs that instance's controller, so step
-// lengths, line searches and stopping tests stay per instance and never wait
-// for the host.  The host enqueues chunks of passes (two in flight), reads
-// back the control blocks once per chunk, retires the instances that have
-// stopped (result copy, slot and memory freed) and admits new ones between
-// chunks; the tile tables are rebuilt whenever the set of instances changes.
-// A chunk can be replayed from a captured hipGraph.
+// of a device pool holding its solver state.  Instances are grouped by
+// algorithm into lanes; a lane has its own stream, tile table and chunk
+// pipeline, so the passes of one algorithm never wait for the other's.  A
+// pass of the gradient lane is one launch of the fused gradient kernel over
+// the tile table of all its instances, a pass of the greedy lane the greedy
+// step and evaluation over all of its; each instance's last block runs that
+// instance's controller, so step lengths, line searches and stopping tests
+// stay per instance and never wait for the host.  Per lane the host enqueues
+// chunks of passes (two in flight), reads back the control blocks once per
+// chunk, retires the instances that have stopped (result copy, slot and
+// memory freed) and admits new ones between chunks; the tile table is
+// rebuilt whenever the lane's set of instances changes.  A chunk can be
+// replayed from a captured hipGraph.
 //
 // An instance's result does not depend on what else is in the batch: its
 // tiles, the order of its partial sums and its controller are its own.
@@ -39,27 +42,49 @@ enum SlotState { SLOT_FREE = 0, SLOT_ACTIVE = 1, SLOT_RETIRING = 2 };
 struct Slot {
     int state = SLOT_FREE;
     long long ticket = -1;
+    int lane = 0;                 // DT_GRADIENT or DT_GREEDY
     Plan P;
     dt_params params{};
     int host_io = 0;
     float *out_field = nullptr;   // where the result goes (caller's buffer)
-    float *dev_field = nullptr;   // buf[0] of the instance
     size_t pool_off = 0, pool_bytes = 0;
-    long long first_seq = 0;      // first chunk that includes this instance
+    long long first_seq = 0;      // first chunk of its lane that includes this instance
     long long launched = 0;       // passes launched over this instance
     int failed = 0;
     Ctrl final_ctrl{};
     Clock::time_point t_submit;
     hipEvent_t ev_ready = nullptr;   // inputs uploaded (host_io)
-    hipEvent_t ev_stop = nullptr;    // st reached the retirement point
+    hipEvent_t ev_stop = nullptr;    // the lane's stream reached the retirement point
     hipEvent_t ev_done = nullptr;    // result in place
 };
 
 struct Chunk {
     long long seq;
-    int snap;        // pinned snapshot buffer
-    int hi;          // slots [0, hi) were snapshotted
+    long long order;  // engine-wide enqueue order
+    int snap;         // pinned snapshot buffer
+    int hi;           // slots [0, hi) were snapshotted
     hipEvent_t ev;
+};
+
+// The instances of one algorithm and their pass pipeline.
+struct Lane {
+    int kind = DT_GRADIENT;
+    hipStream_t st = nullptr;     // passes, instance uploads, result copies
+    hipStream_t cap = nullptr;    // graph capture
+    bool own_st = true;
+    Tile *d_tiles = nullptr;
+    size_t d_cap = 0;
+    Tile *h_tiles[2] = {nullptr, nullptr};   // pinned staging per snapshot buffer
+    size_t h_cap = 0;
+    int ntiles = 0;
+    bool dirty = true;
+    hipGraphExec_t graph = nullptr;
+    int graph_ntiles = -1;
+    Tile *graph_tiles = nullptr;
+    Ctrl *h_snap[2] = {nullptr, nullptr};
+    std::deque<Chunk> inflight;
+    long long seq = 0;
+    int n_active = 0;
 };
 
 // First-fit allocator over [0, size) with coalescing.
@@ -108,6 +133,7 @@ struct Pool {
 
 constexpr size_t POOL_GRAIN = (size_t)1 << 16;
 constexpr int MAX_SLOTS = 4096;
+constexpr int NLANES = 2;
 
 }  // namespace
 
@@ -123,10 +149,8 @@ struct dt_engine {
     int use_graphs = 0;
     int tz_policy = 0;
     int max_slots = 256;
-    hipStream_t st = nullptr, io = nullptr, cap = nullptr;
-    hipStream_t side = nullptr, cap_side = nullptr;   // greedy kernels, forked from / joined to st (cap)
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool own_st = false;
+    hipStream_t io = nullptr;    // host staging copies
+    Lane lanes[NLANES];
 
     char *pool_mem = nullptr;
     Pool pool;
@@ -136,26 +160,12 @@ struct dt_engine {
     unsigned int *d_counter = nullptr;
     Inst *h_insts = nullptr;     // pinned mirror, one per slot
     Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks
-    Ctrl *h_snap[2] = {nullptr, nullptr};
     int *h_zero = nullptr;       // pinned 0 (deactivates a stuck instance)
 
-    // tile tables: [0] gradient pass, [1] greedy kernels
-    Tile *d_tiles[2] = {nullptr, nullptr};
-    size_t d_cap[2] = {0, 0};
-    Tile *h_tiles[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // [staging][kind]
-    size_t h_cap = 0;
-    int ntiles[2] = {0, 0};
-    bool dirty = true;
-
-    hipGraphExec_t graph = nullptr;
-    int graph_ntiles[2] = {-1, -1};
-    Tile *graph_tiles[2] = {nullptr, nullptr};
-
     std::vector<Slot> slots;
-    std::deque<Chunk> inflight;
     std::vector<hipEvent_t> chunk_events;   // free chunk events
     std::vector<dt_engine_result> results;
-    long long seq = 0;
+    long long order = 0;
     long long next_ticket = 0;
     int n_active = 0, n_retiring = 0;
     dt_engine_stats stats{};
@@ -190,135 +200,113 @@ size_t footprint(const dt_engine *e, const Plan &P, int host_io)
     return align_up(b, POOL_GRAIN);
 }
 
-int ensure_tables(dt_engine *e, size_t need)
+int ensure_tables(Lane &L, size_t need)
 {
-    for (int k = 0; k < 2; ++k) {
-        if (e->d_cap[k] < need) {
-            // the current table may still be read by queued passes
-            DT_HIP(hipStreamSynchronize(e->st));
-            if (e->d_tiles[k]) DT_HIP(hipFree(e->d_tiles[k]));
-            e->d_tiles[k] = nullptr;
-            const size_t cap = std::max<size_t>(need, 2 * e->d_cap[k]);
-            DT_HIP(hipMalloc(&e->d_tiles[k], cap * sizeof(Tile)));
-            e->d_cap[k] = cap;
-        }
+    if (L.d_cap < need) {
+        // the current table may still be read by queued passes
+        DT_HIP(hipStreamSynchronize(L.st));
+        if (L.d_tiles) DT_HIP(hipFree(L.d_tiles));
+        L.d_tiles = nullptr;
+        const size_t cap = std::max<size_t>(need, 2 * L.d_cap);
+        DT_HIP(hipMalloc(&L.d_tiles, cap * sizeof(Tile)));
+        L.d_cap = cap;
     }
-    if (e->h_cap < need) {
-        DT_HIP(hipStreamSynchronize(e->st));
-        const size_t cap = std::max<size_t>(need, 2 * e->h_cap);
-        for (int s = 0; s < 2; ++s)
-            for (int k = 0; k < 2; ++k) {
-                if (e->h_tiles[s][k]) DT_HIP(hipHostFree(e->h_tiles[s][k]));
-                e->h_tiles[s][k] = nullptr;
-                DT_HIP(hipHostMalloc((void **)&e->h_tiles[s][k], cap * sizeof(Tile), hipHostMallocDefault));
-            }
-        e->h_cap = cap;
+    if (L.h_cap < need) {
+        DT_HIP(hipStreamSynchronize(L.st));
+        const size_t cap = std::max<size_t>(need, 2 * L.h_cap);
+        for (int s = 0; s < 2; ++s) {
+            if (L.h_tiles[s]) DT_HIP(hipHostFree(L.h_tiles[s]));
+            L.h_tiles[s] = nullptr;
+            DT_HIP(hipHostMalloc((void **)&L.h_tiles[s], cap * sizeof(Tile), hipHostMallocDefault));
+        }
+        L.h_cap = cap;
     }
     return 0;
 }
 
-// Rebuilds and uploads the tile tables of the active instances (staging
-// buffer `stage` is free: the chunk that used it last has completed).
-int rebuild_tables(dt_engine *e, int stage)
+// Rebuilds and uploads the lane's tile table (staging buffer `stage` is free:
+// the chunk that used it last has completed).
+int rebuild_tables(dt_engine *e, Lane &L, int stage)
 {
-    std::vector<std::pair<int, int>> seg[2];
-    size_t count[2] = {0, 0};
+    std::vector<std::pair<int, int>> seg;
+    size_t count = 0;
     for (int s = 0; s < (int)e->slots.size(); ++s) {
         const Slot &sl = e->slots[s];
-        if (sl.state != SLOT_ACTIVE) continue;
-        const int k = sl.P.algorithm == DT_GRADIENT ? 0 : 1;
-        seg[k].push_back({s, sl.P.ntiles});
-        count[k] += (size_t)sl.P.ntiles;
+        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
+        seg.push_back({s, sl.P.ntiles});
+        count += (size_t)sl.P.ntiles;
     }
-    if (ensure_tables(e, std::max<size_t>(std::max(count[0], count[1]), 1))) return -1;
-    for (int k = 0; k < 2; ++k) {
-        std::vector<Tile> t;
-        t.reserve(count[k]);
-        append_tiles_xcd(t, seg[k]);
-        e->ntiles[k] = (int)t.size();
-        if (!t.empty()) {
-            std::memcpy(e->h_tiles[stage][k], t.data(), t.size() * sizeof(Tile));
-            DT_HIP(hipMemcpyAsync(e->d_tiles[k], e->h_tiles[stage][k], t.size() * sizeof(Tile),
-                                  hipMemcpyHostToDevice, e->st));
-        }
+    if (ensure_tables(L, std::max<size_t>(count, 1))) return -1;
+    std::vector<Tile> t;
+    t.reserve(count);
+    append_tiles_xcd(t, seg);
+    L.ntiles = (int)t.size();
+    if (!t.empty()) {
+        std::memcpy(L.h_tiles[stage], t.data(), t.size() * sizeof(Tile));
+        DT_HIP(hipMemcpyAsync(L.d_tiles, L.h_tiles[stage], t.size() * sizeof(Tile), hipMemcpyHostToDevice, L.st));
     }
-    e->dirty = false;
+    L.dirty = false;
     return 0;
 }
 
-// One chunk of passes: the gradient passes on `s`, the greedy kernels on
-// `s2` (forked from `s` and joined back), so the two groups overlap.
-int launch_passes(dt_engine *e, hipStream_t s, hipStream_t s2)
+int launch_passes(dt_engine *e, const Lane &L, hipStream_t s)
 {
-    const bool both = e->ntiles[0] > 0 && e->ntiles[1] > 0;
-    hipStream_t sg = both ? s2 : s;
-    if (both) {
-        DT_HIP(hipEventRecord(e->ev_fork, s));
-        DT_HIP(hipStreamWaitEvent(s2, e->ev_fork, 0));
-    }
     for (int i = 0; i < e->chunk; ++i) {
-        if (e->ntiles[0] > 0)
-            DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, e->d_tiles[0], e->ntiles[0], s));
-        if (e->ntiles[1] > 0) {
-            DT_HIP(launch_greedy_step(e->d_insts, e->d_tiles[1], e->ntiles[1], sg));
-            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, e->d_tiles[1], e->ntiles[1], sg));
+        if (L.kind == DT_GRADIENT) {
+            DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, L.d_tiles, L.ntiles, s));
+        } else {
+            DT_HIP(launch_greedy_step(e->d_insts, L.d_tiles, L.ntiles, s));
+            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, L.d_tiles, L.ntiles, s));
         }
-    }
-    if (both) {
-        DT_HIP(hipEventRecord(e->ev_join, s2));
-        DT_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
     }
     return 0;
 }
 
-int build_graph(dt_engine *e)
+int build_graph(dt_engine *e, Lane &L)
 {
-    if (e->graph) {
-        DT_HIP(hipGraphExecDestroy(e->graph));
-        e->graph = nullptr;
+    if (L.graph) {
+        DT_HIP(hipGraphExecDestroy(L.graph));
+        L.graph = nullptr;
     }
     hipGraph_t g = nullptr;
-    DT_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = launch_passes(e, e->cap, e->cap_side);
-    const hipError_t ce = hipStreamEndCapture(e->cap, &g);
+    DT_HIP(hipStreamBeginCapture(L.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = launch_passes(e, L, L.cap);
+    const hipError_t ce = hipStreamEndCapture(L.cap, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return -1;
     }
     DT_HIP(ce);
-    const hipError_t ie = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+    const hipError_t ie = hipGraphInstantiate(&L.graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     DT_HIP(ie);
-    e->graph_ntiles[0] = e->ntiles[0];
-    e->graph_ntiles[1] = e->ntiles[1];
-    e->graph_tiles[0] = e->d_tiles[0];
-    e->graph_tiles[1] = e->d_tiles[1];
+    L.graph_ntiles = L.ntiles;
+    L.graph_tiles = L.d_tiles;
     e->stats.graph_builds++;
     return 0;
 }
 
-int enqueue_chunk(dt_engine *e)
+int enqueue_chunk(dt_engine *e, Lane &L)
 {
-    const int stage = (int)(e->seq & 1);
-    if (e->dirty && rebuild_tables(e, stage)) return -1;
+    const int stage = (int)(L.seq & 1);
+    if (L.dirty && rebuild_tables(e, L, stage)) return -1;
     if (e->use_graphs) {
-        if (!e->graph || e->graph_ntiles[0] != e->ntiles[0] || e->graph_ntiles[1] != e->ntiles[1] ||
-            e->graph_tiles[0] != e->d_tiles[0] || e->graph_tiles[1] != e->d_tiles[1]) {
-            if (build_graph(e)) return -1;
-        }
-        DT_HIP(hipGraphLaunch(e->graph, e->st));
+        if (!L.graph || L.graph_ntiles != L.ntiles || L.graph_tiles != L.d_tiles)
+            if (build_graph(e, L)) return -1;
+        DT_HIP(hipGraphLaunch(L.graph, L.st));
     } else {
-        if (launch_passes(e, e->st, e->side)) return -1;
+        if (launch_passes(e, L, L.st)) return -1;
     }
     int hi = 0;
     for (int s = 0; s < (int)e->slots.size(); ++s) {
         Slot &sl = e->slots[s];
-        if (sl.state != SLOT_ACTIVE) continue;
+        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
         hi = s + 1;
         sl.launched += e->chunk;
     }
     Chunk c;
-    c.seq = e->seq++;
+    c.seq = L.seq++;
+    c.order = e->order++;
     c.snap = stage;
     c.hi = hi;
     if (e->chunk_events.empty()) {
@@ -328,11 +316,12 @@ int enqueue_chunk(dt_engine *e)
     }
     c.ev = e->chunk_events.back();
     e->chunk_events.pop_back();
+    // the snapshot also copies other lanes' control blocks, which may be in
+    // flux; only this lane's entries are read
     if (hi > 0)
-        DT_HIP(hipMemcpyAsync(e->h_snap[stage], e->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost,
-                              e->st));
-    DT_HIP(hipEventRecord(c.ev, e->st));
-    e->inflight.push_back(c);
+        DT_HIP(hipMemcpyAsync(L.h_snap[stage], e->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost, L.st));
+    DT_HIP(hipEventRecord(c.ev, L.st));
+    L.inflight.push_back(c);
     e->stats.chunks++;
     e->stats.passes += e->chunk;
     return 0;
@@ -342,24 +331,26 @@ int enqueue_chunk(dt_engine *e)
 int retire(dt_engine *e, int s, const Ctrl &fin, int failed)
 {
     Slot &sl = e->slots[s];
+    Lane &L = e->lanes[sl.lane];
     sl.final_ctrl = fin;
     sl.failed = failed;
     sl.state = SLOT_RETIRING;
     e->n_active--;
+    L.n_active--;
     e->n_retiring++;
-    e->dirty = true;
+    L.dirty = true;
     const int rb = (fin.result_buf >= 0 && fin.result_buf < NBUF) ? fin.result_buf : 0;
     const size_t bytes = 3 * (size_t)sl.P.n * sizeof(float);
     float *src = e->h_insts[s].buf[rb];
     if (sl.host_io) {
-        DT_HIP(hipEventRecord(sl.ev_stop, e->st));
+        DT_HIP(hipEventRecord(sl.ev_stop, L.st));
         DT_HIP(hipStreamWaitEvent(e->io, sl.ev_stop, 0));
         DT_HIP(hipMemcpyAsync(sl.out_field, src, bytes, hipMemcpyDeviceToHost, e->io));
         DT_HIP(hipEventRecord(sl.ev_done, e->io));
     } else {
         if (src != sl.out_field)
-            DT_HIP(hipMemcpyAsync(sl.out_field, src, bytes, hipMemcpyDeviceToDevice, e->st));
-        DT_HIP(hipEventRecord(sl.ev_done, e->st));
+            DT_HIP(hipMemcpyAsync(sl.out_field, src, bytes, hipMemcpyDeviceToDevice, L.st));
+        DT_HIP(hipEventRecord(sl.ev_done, L.st));
     }
     return 0;
 }
@@ -402,17 +393,17 @@ int reap(dt_engine *e, bool wait)
     return 0;
 }
 
-int process_chunk(dt_engine *e, const Chunk &c)
+int process_chunk(dt_engine *e, Lane &L, const Chunk &c)
 {
-    const Ctrl *snap = e->h_snap[c.snap];
+    const Ctrl *snap = L.h_snap[c.snap];
     for (int s = 0; s < c.hi; ++s) {
         Slot &sl = e->slots[s];
-        if (sl.state != SLOT_ACTIVE || c.seq < sl.first_seq) continue;
+        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind || c.seq < sl.first_seq) continue;
         if (snap[s].phase == PH_DONE) {
             if (retire(e, s, snap[s], 0)) return -1;
         } else if (sl.launched > max_passes(sl.params)) {
             // stop its passes, then return the last accepted iterate
-            DT_HIP(hipMemcpyAsync(&e->d_ctrl[s].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, e->st));
+            DT_HIP(hipMemcpyAsync(&e->d_ctrl[s].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
             Ctrl f = snap[s];
             f.result_buf = f.base;
             if (retire(e, s, f, 1)) return -1;
@@ -421,26 +412,52 @@ int process_chunk(dt_engine *e, const Chunk &c)
     return 0;
 }
 
+int fill(dt_engine *e)
+{
+    for (Lane &L : e->lanes)
+        while ((int)L.inflight.size() < 2 && L.n_active > 0)
+            if (enqueue_chunk(e, L)) return -1;
+    return 0;
+}
+
+int complete(dt_engine *e, Lane &L)
+{
+    const Chunk c = L.inflight.front();
+    L.inflight.pop_front();
+    e->chunk_events.push_back(c.ev);
+    return process_chunk(e, L, c);
+}
+
 int progress(dt_engine *e, int wait)
 {
     if (reap(e, false)) return -1;
-    while ((int)e->inflight.size() < 2 && e->n_active > 0)
-        if (enqueue_chunk(e)) return -1;
-    if (!e->inflight.empty()) {
-        const Chunk c = e->inflight.front();
-        hipError_t q = wait ? hipEventSynchronize(c.ev) : hipEventQuery(c.ev);
-        if (q != hipErrorNotReady) {
+    if (fill(e)) return -1;
+    // finished chunks of every lane, without blocking
+    bool any = false;
+    for (Lane &L : e->lanes) {
+        while (!L.inflight.empty()) {
+            const hipError_t q = hipEventQuery(L.inflight.front().ev);
+            if (q == hipErrorNotReady) break;
             DT_HIP(q);
-            e->inflight.pop_front();
-            e->chunk_events.push_back(c.ev);
-            if (process_chunk(e, c)) return -1;
-            // keep the device busy while the host handles results
-            while ((int)e->inflight.size() < 2 && e->n_active > 0)
-                if (enqueue_chunk(e)) return -1;
+            if (complete(e, L)) return -1;
+            any = true;
         }
-    } else if (wait && e->n_retiring > 0) {
-        if (reap(e, true)) return -1;
     }
+    if (!any && wait) {
+        // block on the chunk enqueued first
+        Lane *first = nullptr;
+        for (Lane &L : e->lanes)
+            if (!L.inflight.empty() && (!first || L.inflight.front().order < first->inflight.front().order))
+                first = &L;
+        if (first) {
+            DT_HIP(hipEventSynchronize(first->inflight.front().ev));
+            if (complete(e, *first)) return -1;
+        } else if (e->n_retiring > 0) {
+            if (reap(e, true)) return -1;
+        }
+    }
+    // keep the device busy while the host handles results
+    if (fill(e)) return -1;
     if (reap(e, false)) return -1;
     return e->n_active + e->n_retiring;
 }
@@ -449,24 +466,26 @@ void destroy(dt_engine *e)
 {
     if (!e) return;
     (void)hipSetDevice(e->device);
-    if (e->st) (void)hipStreamSynchronize(e->st);
+    for (Lane &L : e->lanes)
+        if (L.st) (void)hipStreamSynchronize(L.st);
     if (e->io) (void)hipStreamSynchronize(e->io);
-    if (e->side) (void)hipStreamSynchronize(e->side);
-    if (e->graph) (void)hipGraphExecDestroy(e->graph);
-    for (auto &c : e->inflight) (void)hipEventDestroy(c.ev);
+    for (Lane &L : e->lanes) {
+        if (L.graph) (void)hipGraphExecDestroy(L.graph);
+        for (auto &c : L.inflight) (void)hipEventDestroy(c.ev);
+        if (L.d_tiles) (void)hipFree(L.d_tiles);
+        for (int s = 0; s < 2; ++s) {
+            if (L.h_tiles[s]) (void)hipHostFree(L.h_tiles[s]);
+            if (L.h_snap[s]) (void)hipHostFree(L.h_snap[s]);
+        }
+        if (L.cap) (void)hipStreamDestroy(L.cap);
+        if (L.own_st && L.st) (void)hipStreamDestroy(L.st);
+    }
     for (auto ev : e->chunk_events) (void)hipEventDestroy(ev);
     for (auto &sl : e->slots) {
         if (sl.ev_ready) (void)hipEventDestroy(sl.ev_ready);
         if (sl.ev_stop) (void)hipEventDestroy(sl.ev_stop);
         if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
     }
-    for (int k = 0; k < 2; ++k)
-        if (e->d_tiles[k]) (void)hipFree(e->d_tiles[k]);
-    for (int s = 0; s < 2; ++s)
-        for (int k = 0; k < 2; ++k)
-            if (e->h_tiles[s][k]) (void)hipHostFree(e->h_tiles[s][k]);
-    for (int s = 0; s < 2; ++s)
-        if (e->h_snap[s]) (void)hipHostFree(e->h_snap[s]);
     if (e->h_insts) (void)hipHostFree(e->h_insts);
     if (e->h_ctrl0) (void)hipHostFree(e->h_ctrl0);
     if (e->h_zero) (void)hipHostFree(e->h_zero);
@@ -474,13 +493,7 @@ void destroy(dt_engine *e)
     if (e->d_ctrl) (void)hipFree(e->d_ctrl);
     if (e->d_counter) (void)hipFree(e->d_counter);
     if (e->pool_mem) (void)hipFree(e->pool_mem);
-    if (e->cap) (void)hipStreamDestroy(e->cap);
-    if (e->cap_side) (void)hipStreamDestroy(e->cap_side);
-    if (e->side) (void)hipStreamDestroy(e->side);
-    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
-    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->io) (void)hipStreamDestroy(e->io);
-    if (e->own_st && e->st) (void)hipStreamDestroy(e->st);
     delete e;
 }
 
@@ -503,18 +516,21 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     budget = budget / POOL_GRAIN * POOL_GRAIN;
     if (budget < POOL_GRAIN) DT_FAIL("hbm_budget too small");
 
-    if (stream) {
-        e->st = (hipStream_t)stream;
-    } else {
-        DT_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
-        e->own_st = true;
+    const int S = e->max_slots;
+    for (int k = 0; k < NLANES; ++k) {
+        Lane &L = e->lanes[k];
+        L.kind = k;
+        if (k == DT_GRADIENT && stream) {
+            L.st = (hipStream_t)stream;
+            L.own_st = false;
+        } else {
+            DT_HIP(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+        }
+        DT_HIP(hipStreamCreateWithFlags(&L.cap, hipStreamNonBlocking));
+        for (int s = 0; s < 2; ++s)
+            DT_HIP(hipHostMalloc((void **)&L.h_snap[s], (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
     }
     DT_HIP(hipStreamCreateWithFlags(&e->io, hipStreamNonBlocking));
-    DT_HIP(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
-    DT_HIP(hipStreamCreateWithFlags(&e->cap_side, hipStreamNonBlocking));
-    DT_HIP(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
-    DT_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
-    DT_HIP(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming));
 
     if (hipMalloc(&e->pool_mem, budget) != hipSuccess) {
         (void)hipGetLastError();
@@ -522,7 +538,6 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
         DT_FAIL("cannot allocate the engine's device pool (hbm_budget)");
     }
     e->pool.init(budget);
-    const int S = e->max_slots;
     DT_HIP(hipMalloc(&e->d_insts, (size_t)S * sizeof(Inst)));
     DT_HIP(hipMalloc(&e->d_ctrl, (size_t)S * sizeof(Ctrl)));
     DT_HIP(hipMalloc(&e->d_counter, (size_t)S * sizeof(unsigned int)));
@@ -532,8 +547,6 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
     DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
     *e->h_zero = 0;
-    for (int s = 0; s < 2; ++s)
-        DT_HIP(hipHostMalloc((void **)&e->h_snap[s], (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
     e->slots.resize(S);
     for (auto &sl : e->slots) {
         DT_HIP(hipEventCreateWithFlags(&sl.ev_ready, hipEventDisableTiming));
@@ -560,7 +573,9 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
     size_t off = 0;
     if (!e->pool.alloc(bytes, off)) return DT_ENGINE_FULL;
 
+    Lane &L = e->lanes[algorithm];
     Slot &sl = e->slots[s];
+    sl.lane = algorithm;
     sl.P = P;
     sl.params = *params;
     sl.host_io = host_io;
@@ -583,7 +598,7 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
             hipMemcpyAsync(t, target, (size_t)n * sizeof(float), hipMemcpyHostToDevice, e->io) != hipSuccess ||
             hipMemcpyAsync(m, mask, (size_t)n, hipMemcpyHostToDevice, e->io) != hipSuccess ||
             hipEventRecord(sl.ev_ready, e->io) != hipSuccess ||
-            hipStreamWaitEvent(e->st, sl.ev_ready, 0) != hipSuccess) {
+            hipStreamWaitEvent(L.st, sl.ev_ready, 0) != hipSuccess) {
             set_error(__LINE__, __FILE__, __func__, "cannot stage the instance's inputs on the device");
             rc = -1;
         }
@@ -595,23 +610,22 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
         d_mask = (const uint8_t *)mask;
     }
     sl.out_field = field;
-    sl.dev_field = d_field;
     if (!rc) {
         e->h_insts[s] = make_inst(P, dx, dy, dz, *params, d_target, d_mask, d_field, cv, nullptr, 0,
                                   &e->d_ctrl[s], &e->d_counter[s]);
         e->h_ctrl0[s] = initial_ctrl(algorithm, *params);
-        if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, e->st) !=
+        if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||
-            hipMemcpyAsync(&e->d_ctrl[s], &e->h_ctrl0[s], sizeof(Ctrl), hipMemcpyHostToDevice, e->st) !=
+            hipMemcpyAsync(&e->d_ctrl[s], &e->h_ctrl0[s], sizeof(Ctrl), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||
-            hipMemsetAsync(&e->d_counter[s], 0, sizeof(unsigned int), e->st) != hipSuccess) {
+            hipMemsetAsync(&e->d_counter[s], 0, sizeof(unsigned int), L.st) != hipSuccess) {
             set_error(__LINE__, __FILE__, __func__, "cannot upload the instance record");
             rc = -1;
         }
         // the greedy step never writes the outer face of its output buffer,
         // which starts at zero (src/disptools.c:61, displacement_field_greedy.c:43-65)
         if (!rc && algorithm == DT_GREEDY &&
-            hipMemsetAsync(e->h_insts[s].buf[1], 0, 3 * (size_t)n * sizeof(float), e->st) != hipSuccess) {
+            hipMemsetAsync(e->h_insts[s].buf[1], 0, 3 * (size_t)n * sizeof(float), L.st) != hipSuccess) {
             set_error(__LINE__, __FILE__, __func__, "cannot clear the greedy buffer");
             rc = -1;
         }
@@ -622,10 +636,11 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
     }
     sl.state = SLOT_ACTIVE;
     sl.ticket = e->next_ticket++;
-    sl.first_seq = e->seq;
+    sl.first_seq = L.seq;
     sl.launched = 0;
     e->n_active++;
-    e->dirty = true;
+    L.n_active++;
+    L.dirty = true;
     e->stats.admitted++;
     return sl.ticket;
 }

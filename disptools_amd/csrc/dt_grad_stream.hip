@@ -74,11 +74,14 @@ __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 // Stage A at one point: J - 1, error, weight and the nine weighted cofactors
 // W[3 r + a] = w * M_{a, r} (r: relation axis x, y, z).
 template <typename CT>
-__device__ __forceinline__ void stream_stage_a(const PassParams &P, int zz, int gx, int gy,
+__device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, int zz, int gx, int gy,
                                                const float (&sm)[3][3][4], const float (&v0)[3], float T,
                                                bool in_mask, bool count, double &Esum, float &mx, CT (&W)[9])
 {
-    const float dJ = jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
+    // planes away from the z faces: the column's own coefficients (no
+    // wave-wide branch on boundary lanes); the four face planes: region map
+    const float dJ = (zz >= 2 && zz <= P.nz - 3) ? jacobian_delta_coef<CT>(K, sm, v0)
+                                                : jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     if (count) {
@@ -154,6 +157,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const bool pin = what != 3 && gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
     const bool col_interior = own && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
     const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
+    const ColCoef K = column_coefficients(P, gx, gy);
     const int pi = what == 3 ? -1 : (ly + 3) * HX + (lx + 3);
 
     // halo slots: a share of rings 2-3 and the ring-1 corners
@@ -282,7 +286,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             }
             const bool count = own && zz >= z0 && zz < z1;
             CT W[9];
-            stream_stage_a<CT>(P, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Esum, mx, W);
+            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Esum, mx, W);
             const int pz = zz & 1;
             if (what != 2) {
 #pragma unroll

@@ -80,6 +80,53 @@ __device__ __forceinline__ float jacobian_delta(const PassParams &P, int gx, int
     return (float)det_plus_identity_minus_one<CT>(d4);
 }
 
+// Stencil coefficients of a column (gx, gy) on the planes away from the z
+// faces (2 <= z <= nz - 3), where the region map depends on x and y alone.
+// Every stencil kind is  d = A (p1 - m1) + B (p2 - m2) + G c  (1/spacing
+// folded in), because samples outside the volume are zero:
+//   4th-order central  A = 2/3,  B = -1/12, G = 0
+//   2nd-order central  A = 1/2,  B = 0,     G = 0
+//   forward  (m1 = m2 = 0)  A = 2, B = -1/2, G = -3/2   (-3/2 c + 2 p1 - 1/2 p2)
+//   backward (p1 = p2 = 0)  A = 2, B = -1/2, G = +3/2   (3/2 c - 2 m1 + 1/2 m2)
+struct ColCoef {
+    float A[3], B[3], G[3];
+};
+
+__device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, int gx, int gy)
+{
+    int kind[3];
+    classify(gx, gy, 2, P.nx, P.ny, 5, kind);   // a plane away from the z faces
+    ColCoef K;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int k = kind[a];
+        const float inv = P.inv[a];
+        K.A[a] = k == K_C4 ? P.c4a[a] : (k == K_C2 ? P.c2[a] : 2.0f * inv);
+        K.B[a] = k == K_C4 ? -P.c4b[a] : (k == K_C2 ? 0.0f : -0.5f * inv);
+        K.G[a] = k == K_FWD ? -1.5f * inv : (k == K_BWD ? 1.5f * inv : 0.0f);
+    }
+    return K;
+}
+
+// J - 1 from the column's coefficients (planes 2 .. nz - 3).  A lane with the
+// 4th-order stencil rounds as jacobian_delta's interior form (G c = 0 adds
+// an exact zero).
+template <typename CT>
+__device__ __forceinline__ float jacobian_delta_coef(const ColCoef &K, const float (&sm)[3][3][4],
+                                                     const float (&v0)[3])
+{
+    CT d4[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const CT D1 = (CT)(sm[c][a][2] - sm[c][a][1]);
+            const CT D2 = (CT)(sm[c][a][3] - sm[c][a][0]);
+            d4[c][a] = __builtin_fma(D1, (CT)K.A[a], __builtin_fma(D2, (CT)K.B[a], (CT)K.G[a] * (CT)v0[c]));
+        }
+    return (float)det_plus_identity_minus_one<CT>(d4);
+}
+
 // Penalised weight of the adjoint stencil (displacement_field_gradient.c:71-88):
 // w = e + zeta (J - delta) where J < delta.
 __device__ __forceinline__ float penalised_weight(float e, float J, const PassParams &P)

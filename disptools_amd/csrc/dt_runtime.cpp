@@ -209,10 +209,10 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident)
     return best;
 }
 
-// Default pass variant: the single-barrier streaming pass on 32 x 16 column
+// Default pass variant: the single-barrier streaming pass on 64 x 12 column
 // tiles with the tile ring in waves of its own (measured fastest at 128^3
-// among the compiled variants, with 64 x 12); DISPTOOLS_GRAD_VARIANT overrides it.
-constexpr int DEFAULT_GRAD_VARIANT = STREAM_BASE;
+// among the compiled variants); DISPTOOLS_GRAD_VARIANT overrides it.
+constexpr int DEFAULT_GRAD_VARIANT = STREAM_BASE + 4;
 
 int variant_from_env()
 {

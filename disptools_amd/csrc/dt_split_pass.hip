@@ -295,7 +295,7 @@ __device__ __forceinline__ void update_body(UpdSmem<TX, TY> &S, const Inst &I, c
                                             double &Gsum)
 {
     using SM = UpdSmem<TX, TY>;
-    constexpr int HX = SM::HX, PL = SM::PL, WX = SM::WX, NT = TX * TY;
+    constexpr int HX = SM::HX, PL = SM::PL, WX = SM::WX;
     constexpr int NHALO = 2 * TX + 2 * TY;
     using LD = PlaneLoader<TX, TY, 2, INTERP>;
     constexpr int PER = LD::PER;
