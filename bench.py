@@ -57,6 +57,8 @@ def parse():
                     help="requests of the serving stream (0 to skip it)")
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--graphs", type=int, default=0, help="engine replays chunks from hipGraphs")
+    ap.add_argument("--slots", type=int, default=0, help="engine slots (instances solved at once; 0: 256)")
+    ap.add_argument("--serve-dump", default="", help="write per-request records of the serving run (JSON)")
     return ap.parse_args()
 
 
@@ -201,7 +203,7 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def serve_bench(count, qps, it_max, graphs, seed=7):
+def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
     """Open-loop serving stream: Poisson arrivals at `qps`, cube side
     log-normal (median 96, sigma_ln 0.35, clipped to [32, 256]), 80 % gradient
     and 20 % greedy requests, targets and masks of the 128^3 workload's
@@ -263,7 +265,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
     vi_b = 0
     for d in data:
         d[2].zero_()
-    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, max_slots=slots) as eng:
         queue = collections.deque()
         tickets = {}
         nxt = 0
@@ -296,12 +298,18 @@ def serve_bench(count, qps, it_max, graphs, seed=7):
                     time.sleep(min(wait, 0.001))
         batch_span = time.perf_counter() - start - reqs[0]["arrival"]
         stats = eng.stats()
+    if dump:
+        with open(dump, "w") as fh:
+            json.dump([dict(side=r["side"], algorithm=r["algorithm"], arrival=r["arrival"],
+                            plain_latency=lp, batched_latency=lb, iterations=ip["iterations"])
+                       for r, lp, lb, ip in zip(reqs, lat_plain, lat_b, info_plain)], fh)
     same_reason = sum(1 for a, b in zip(info_plain, info_b) if a["reason"] == b["reason"])
     diff = max(abs(a["final_max_error"] - b["final_max_error"]) for a, b in zip(info_plain, info_b))
     return {
         "workload": "%d requests, Poisson %.1f req/s, cube side log-normal (median 96, sigma_ln 0.35, "
                     "clip [32, 256]), 80%% gradient / 20%% greedy, it_max=%d, host buffers (pinned)"
                     % (len(reqs), qps, it_max),
+        "engine_slots": slots or 256,
         "voxels": sum(voxels),
         "plain": {"p50_s": _pct(lat_plain, 50), "p99_s": _pct(lat_plain, 99),
                   "mean_s": float(np.mean(lat_plain)), "voxel_iters_per_s": vi_plain / plain_span,
@@ -330,7 +338,8 @@ def main():
         if args.mode == "batch":
             print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs)))
         else:
-            print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs)))
+            print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
+                                         slots=args.slots, dump=args.serve_dump)))
         return
 
     import numpy as np
@@ -385,7 +394,8 @@ def main():
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
     batch = batch_bench(args.batch, args.it_max, args.graphs) if args.batch > 0 else None
-    serving = serve_bench(args.serve, args.qps, args.it_max, args.graphs) if args.serve > 0 else None
+    serving = (serve_bench(args.serve, args.qps, args.it_max, args.graphs, slots=args.slots)
+               if args.serve > 0 else None)
 
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     cpu = cpu_baseline(T, M, sp, args.cpu_iters, threads) if args.cpu_iters > 0 else None

@@ -1,0 +1,56 @@
+"""Serving wire format: request / response encode-decode round trips (CPU)."""
+
+import numpy as np
+import pytest
+
+from disptools_amd.serving import (SolveRequest, SolveResponse, decode_request, decode_response,
+                                   encode_request, encode_response)
+
+
+def _request(with_field):
+    rng = np.random.default_rng(5)
+    T = (1.0 + 0.05 * rng.normal(size=(4, 5, 6))).astype(np.float32)
+    M = rng.random((4, 5, 6)) < 0.5
+    F = rng.normal(size=(3, 4, 5, 6)).astype(np.float32) if with_field else None
+    return SolveRequest(target=T, mask=M, spacing=(1.0, 1.2, 0.8), algorithm="greedy",
+                        params={"epsilon": 1e-3, "it_max": 77, "strict": True, "zeta": 10.0},
+                        field=F, request_id="r-7")
+
+
+@pytest.mark.parametrize("with_field", [False, True])
+def test_request_round_trip(with_field):
+    r = _request(with_field)
+    d = decode_request(encode_request(r))
+    np.testing.assert_array_equal(d.target, r.target)
+    np.testing.assert_array_equal(d.mask, r.mask)
+    assert d.mask.dtype == np.bool_ and d.target.dtype == np.float32
+    assert d.spacing == r.spacing and d.algorithm == "greedy" and d.request_id == "r-7"
+    assert d.params == {"epsilon": 1e-3, "it_max": 77, "strict": True, "zeta": 10.0}
+    if with_field:
+        np.testing.assert_array_equal(d.field, r.field)
+    else:
+        assert d.field is None
+
+
+def test_response_round_trip():
+    f = np.arange(3 * 2 * 3 * 4, dtype=np.float32).reshape(3, 2, 3, 4)
+    r = SolveResponse(request_id="x", status=0, field=f, reason=5, reason_text="reached desired tolerance",
+                      iterations=12, evaluations=15, rejections=2, final_error=0.5, final_max_error=1e-3,
+                      seconds=0.01)
+    d = decode_response(encode_response(r))
+    np.testing.assert_array_equal(d.field, f)
+    assert (d.reason, d.iterations, d.evaluations, d.rejections) == (5, 12, 15, 2)
+    assert d.final_max_error == 1e-3 and d.reason_text == r.reason_text
+
+
+def test_malformed_messages():
+    good = encode_request(_request(True))
+    with pytest.raises(ValueError):
+        decode_request(good[:-4])
+    with pytest.raises(ValueError):
+        decode_request(b"DTRS1\n{}\n")
+    with pytest.raises(ValueError):
+        encode_request(SolveRequest(target=np.ones((3, 3, 3), np.float32), mask=np.ones((3, 3, 2), bool)))
+    with pytest.raises(ValueError):
+        encode_request(SolveRequest(target=np.ones((3, 3, 3), np.float32), mask=np.ones((3, 3, 3), bool),
+                                    params={"learning_rate": 1.0}))
