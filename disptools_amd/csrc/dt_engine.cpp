@@ -26,6 +26,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "disptools_hip.h"
@@ -444,14 +445,25 @@ int progress(dt_engine *e, int wait)
         }
     }
     if (!any && wait) {
-        // block on the chunk enqueued first
-        Lane *first = nullptr;
-        for (Lane &L : e->lanes)
-            if (!L.inflight.empty() && (!first || L.inflight.front().order < first->inflight.front().order))
-                first = &L;
-        if (first) {
-            DT_HIP(hipEventSynchronize(first->inflight.front().ev));
-            if (complete(e, *first)) return -1;
+        // wait for whichever lane finishes a chunk first (a blocking wait on
+        // one lane would leave the other idle once its chunks have drained)
+        bool pending = false;
+        for (Lane &L : e->lanes) pending |= !L.inflight.empty();
+        if (pending) {
+            for (int spin = 0; !any; ++spin) {
+                for (Lane &L : e->lanes) {
+                    if (L.inflight.empty()) continue;
+                    const hipError_t q = hipEventQuery(L.inflight.front().ev);
+                    if (q == hipErrorNotReady) continue;
+                    DT_HIP(q);
+                    if (complete(e, L)) return -1;
+                    any = true;
+                }
+                if (!any) {
+                    if (spin < 64) std::this_thread::yield();
+                    else std::this_thread::sleep_for(std::chrono::microseconds(20));
+                }
+            }
         } else if (e->n_retiring > 0) {
             if (reap(e, true)) return -1;
         }

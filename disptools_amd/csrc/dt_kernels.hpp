@@ -52,6 +52,10 @@ int split_pass_blocks_per_cu(int precision, int variant);
 hipError_t launch_stream_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                               hipStream_t st);
 int stream_pass_blocks_per_cu(int precision, int variant);
+hipError_t launch_smooth_axis(const float *in, float *out, int nx, int ny, int nz, int ncomp, int axis,
+                              const float *d_w, int radius, hipStream_t st);
+hipError_t launch_resample(const float *in, float *out, int nx, int ny, int nz, int mx, int my, int mz, int ncomp,
+                           float rx, float ry, float rz, int nearest, hipStream_t st);
 hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 

@@ -157,6 +157,10 @@ class TuneResult(ctypes.Structure):
                 ("variant", ctypes.c_int * TUNE_MAX), ("us_per_pass", ctypes.c_double * TUNE_MAX)]
 
 
+lib.dt_scale_volume_device.argtypes = [_sz, _sz, _sz, _sz, _f, _f, _f, _vp, _sz, _sz, _sz, _f, ctypes.c_int, _vp,
+                                       _vp]
+lib.dt_scale_volume_device.restype = ctypes.c_int
+
 lib.dt_tune_gradient.argtypes = [_sz, _sz, _sz, _f, _f, _f, ctypes.c_int, ctypes.POINTER(TuneResult)]
 lib.dt_tune_gradient.restype = ctypes.c_int
 
@@ -267,6 +271,16 @@ def solve_device(algorithm, nx, ny, nz, spacing, d_target, d_mask, d_field, stre
     if rc != 0:
         raise HipError(last_error())
     return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def scale_volume_device(nx, ny, nz, ncomp, spacing, d_in, size_out, sigma, nearest, d_out, stream=0):
+    """Gaussian smoothing (sigma in physical units) + resampling of a device
+    volume [ncomp][nz][ny][nx] onto a grid of size_out = (mx, my, mz)."""
+    mx, my, mz = size_out
+    rc = lib.dt_scale_volume_device(nx, ny, nz, ncomp, *map(float, spacing), d_in, mx, my, mz, float(sigma),
+                                    1 if nearest else 0, d_out, stream or None)
+    if rc != 0:
+        raise HipError(last_error())
 
 
 def tune_gradient(nx, ny, nz, spacing=(1.0, 1.0, 1.0), passes=0):

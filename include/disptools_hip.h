@@ -143,6 +143,18 @@ int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
                   const dt_params *params, float *field,
                   dt_solve_info *info, double *trace, long long trace_cap);
 
+/* Scale step of the image pyramid (the reference's drawing.scale_image,
+ * disptools/drawing.py:23-69, there with SimpleITK): smooths a
+ * [ncomp][nz][ny][nx] float volume with spacing (dx, dy, dz) by a Gaussian of
+ * standard deviation `sigma` in physical units along each axis (taps out to
+ * 4 sigma, edge samples replicated), then resamples it onto an mx x my x mz
+ * grid with the same origin and physical extent (spacing d * n / m): linear
+ * interpolation, or nearest neighbour when `nearest` (masks), 0 outside the
+ * input.  Device pointers; runs on `stream`.  Returns 0 or -1. */
+int dt_scale_volume_device(size_t nx, size_t ny, size_t nz, size_t ncomp, float dx, float dy, float dz,
+                           const float *d_in, size_t mx, size_t my, size_t mz, float sigma, int nearest,
+                           float *d_out, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Autotuner of the gradient pass (no counterpart in the reference).        */
 /* ------------------------------------------------------------------------ */

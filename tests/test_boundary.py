@@ -85,10 +85,13 @@ def test_extension_rejects_matching_on_device():
         ext.displacement(*_disp_args(algorithm="matching"))
 
 
-def test_levels_above_one_not_reproduced():
+def test_pyramid_schedule():
+    """Level sizes halve (reference displacements.py:454-458); it_max lists
+    must have one entry per level (:461-465)."""
     from disptools_amd import displacements
-    with pytest.raises(NotImplementedError):
-        displacements.displacement(np.ones((8, 8, 8), np.float32), levels=2, gpu_id=0)
+    assert displacements.pyramid_sizes((42, 40, 37), 3) == [(42, 40, 37), (21, 20, 18), (10, 10, 9)]
+    with pytest.raises(ValueError):
+        displacements.displacement(np.ones((8, 8, 8), np.float32), levels=2, it_max=[5], gpu_id=0)
 
 
 def test_redistribute_volume_change_closed_form():
