@@ -38,6 +38,7 @@ HIP_FLAGS = [
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_grad_pass.hip", "-ffp-contract=fast"),
                ("dt_split_pass.hip", "-ffp-contract=fast"),
+               ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_pyramid.hip", "-ffp-contract=off"),
                ("dt_runtime.cpp", "-ffp-contract=off"),
                ("dt_engine.cpp", "-ffp-contract=off")]

@@ -277,12 +277,14 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m
 }
 
 // Greedy solver control flow (src/displacement_field_greedy.c:149-238).
-static __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float m)
+// `step_io` is the field volumes a pass with the step moves (step kernel +
+// evaluation kernel: 3; fused pass: 2); the evaluation of the initial guess
+// reads one.
+static __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float m, int step_io)
 {
     const dt_params &p = I.p;
     C->evaluations++;
-    // step: read u, write u; evaluation: read u (error maps are counted by the caller)
-    C->field_io += C->phase == PH_INIT ? 1 : 3;
+    C->field_io += C->phase == PH_INIT ? 1 : step_io;
     const int e_written = C->e_new;
     if (C->phase == PH_INIT) {
         C->E0 = E; C->m0 = m;

@@ -145,7 +145,8 @@ using namespace dt;
 struct dt_engine {
     int device = 0;
     int precision = 32;
-    int variant = 1;
+    int variant = 1;             // gradient pass variant
+    int greedy_variant = 1;      // greedy pass variant
     int chunk = 4;
     int use_graphs = 0;
     int tz_policy = 0;
@@ -185,7 +186,8 @@ int engine_tz(const dt_engine *e, int nz)
 
 Plan engine_plan(const dt_engine *e, int algorithm, int nx, int ny, int nz)
 {
-    return make_plan(algorithm, nx, ny, nz, e->precision, e->variant, engine_tz(e, nz));
+    return make_plan(algorithm, nx, ny, nz, e->precision, algorithm == DT_GRADIENT ? e->variant : e->greedy_variant,
+                     engine_tz(e, nz));
 }
 
 size_t io_bytes(long long n)
@@ -256,8 +258,7 @@ int launch_passes(dt_engine *e, const Lane &L, hipStream_t s)
         if (L.kind == DT_GRADIENT) {
             DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, L.d_tiles, L.ntiles, s));
         } else {
-            DT_HIP(launch_greedy_step(e->d_insts, L.d_tiles, L.ntiles, s));
-            DT_HIP(launch_greedy_eval(e->precision, e->d_insts, L.d_tiles, L.ntiles, s));
+            DT_HIP(launch_greedy_pass(e->precision, e->greedy_variant, e->d_insts, L.d_tiles, L.ntiles, s));
         }
     }
     return 0;
@@ -514,6 +515,7 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipGetDevice(&e->device));
     e->precision = precision_from_env();
     e->variant = (cfg && cfg->variant >= 0 && cfg->variant < GRAD_VARIANTS) ? cfg->variant : variant_from_env();
+    e->greedy_variant = greedy_variant_from_env();
     e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 4;
     e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
     e->tz_policy = cfg ? cfg->tz : 0;

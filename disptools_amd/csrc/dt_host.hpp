@@ -56,7 +56,7 @@ struct Carver {
 struct Plan {
     int algorithm = DT_GRADIENT;
     int precision = 32;      // stencil arithmetic: 32 or 64 bits
-    int variant = 0;         // gradient pass variant (column tile shape)
+    int variant = 0;         // pass variant of the algorithm (column tile shape)
     int nx = 0, ny = 0, nz = 0;
     int tx = TILE_X, ty = TILE_Y, tz = 1;
     int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
@@ -67,11 +67,13 @@ struct Plan {
 int resident_blocks(int precision, int algorithm, int variant);
 // z-chunk depth that minimises (rounds of resident blocks) x (TZ + 3).
 int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident);
-// DISPTOOLS_GRAD_VARIANT / DISPTOOLS_STENCIL_PRECISION overrides.
+// DISPTOOLS_GRAD_VARIANT / DISPTOOLS_GREEDY_VARIANT / DISPTOOLS_STENCIL_PRECISION overrides.
 int variant_from_env();
+int greedy_variant_from_env();
 int precision_from_env();
-// Tiling of an instance; tz <= 0 picks the depth for a launch of this
-// instance alone (choose_tz).
+// Tiling of an instance; `variant` is the pass variant of the algorithm
+// (gradient: GRAD_VARIANTS, greedy: 0 or GREEDY_FUSED); tz <= 0 picks the
+// depth for a launch of this instance alone (choose_tz).
 Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant, int tz);
 
 // Device bytes of an instance's solver state besides target, mask and the

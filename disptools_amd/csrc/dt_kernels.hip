@@ -121,7 +121,7 @@ k_greedy_eval(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
         float m;
         fold_partials<NT>(I, E, m, G, red);
         if (threadIdx.x == 0) {
-            greedy_controller(I, C, E, m);
+            greedy_controller(I, C, E, m, 3);
             __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }

@@ -19,8 +19,6 @@
 
 namespace dt {
 
-constexpr int MAX_SMOOTH_TAPS = 257;
-
 __global__ void __launch_bounds__(256)
 k_smooth_axis(const float *__restrict__ in, float *__restrict__ out, int nx, int ny, int nz, int ncomp, int axis,
               const float *__restrict__ w, int radius)

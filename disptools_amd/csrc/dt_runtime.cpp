@@ -189,7 +189,8 @@ int resident_blocks(int precision, int algorithm, int variant)
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
-    const int per_cu = algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(precision, variant) : 4;
+    const int per_cu = algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(precision, variant)
+                                                : greedy_pass_blocks_per_cu(precision, variant);
     return cus * per_cu;
 }
 
@@ -222,6 +223,14 @@ int variant_from_env()
     return (v >= 0 && v < GRAD_VARIANTS) ? v : DEFAULT_GRAD_VARIANT;
 }
 
+// Greedy pass variant: the fused streaming kernel unless
+// DISPTOOLS_GREEDY_VARIANT=0 asks for the step + evaluation kernels.
+int greedy_variant_from_env()
+{
+    const char *s = std::getenv("DISPTOOLS_GREEDY_VARIANT");
+    return (s && std::strcmp(s, "0") == 0) ? 0 : GREEDY_FUSED;
+}
+
 // Variants the autotuner times: the fused passes (the two-kernel variants
 // need a weight map and are slower everywhere measured).
 static const int TUNE_CANDIDATES[] = {1, STREAM_BASE, STREAM_BASE + 1, STREAM_BASE + 2, STREAM_BASE + 3,
@@ -248,11 +257,11 @@ Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant
     Plan P;
     P.algorithm = algorithm;
     P.precision = precision;
-    P.variant = algorithm == DT_GRADIENT ? variant : 0;
+    P.variant = algorithm == DT_GRADIENT ? variant : (variant == GREEDY_FUSED ? GREEDY_FUSED : 0);
     P.nx = nx; P.ny = ny; P.nz = nz;
     P.n = (long long)nx * ny * nz;
-    P.tx = TILE_X; P.ty = TILE_Y;
     if (algorithm == DT_GRADIENT) grad_variant_dims(P.variant, P.tx, P.ty);
+    else greedy_variant_dims(P.variant, P.tx, P.ty);
     P.tz = tz > 0 ? std::min(tz, nz)
                   : choose_tz(nx, ny, nz, P.tx, P.ty, resident_blocks(precision, algorithm, P.variant));
     P.ntx = (nx + P.tx - 1) / P.tx;
@@ -410,11 +419,12 @@ static int tune_gradient(int nx, int ny, int nz, float dx, float dy, float dz, i
     return 0;
 }
 
-// Gradient-pass variant of a solve: DISPTOOLS_GRAD_VARIANT, else the tuned
-// variant of the shape when DISPTOOLS_AUTOTUNE=1, else the default.
+// Pass variant of a solve.  Gradient: DISPTOOLS_GRAD_VARIANT, else the tuned
+// variant of the shape when DISPTOOLS_AUTOTUNE=1, else the default; greedy:
+// the fused pass unless DISPTOOLS_GREEDY_VARIANT=0.
 static int solve_variant(int algorithm, int nx, int ny, int nz, float dx, float dy, float dz)
 {
-    if (algorithm != DT_GRADIENT) return 0;
+    if (algorithm != DT_GRADIENT) return greedy_variant_from_env();
     if (std::getenv("DISPTOOLS_GRAD_VARIANT")) return variant_from_env();
     const char *t = std::getenv("DISPTOOLS_AUTOTUNE");
     if (!t || std::strcmp(t, "1") != 0) return DEFAULT_GRAD_VARIANT;
@@ -480,8 +490,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
             if (algorithm == DT_GRADIENT) {
                 DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, st));
             } else {
-                DT_HIP(launch_greedy_step(d_inst, d_tiles, ntiles, st));
-                DT_HIP(launch_greedy_eval(precision, d_inst, d_tiles, ntiles, st));
+                DT_HIP(launch_greedy_pass(precision, variant, d_inst, d_tiles, ntiles, st));
             }
         }
         launched += k;

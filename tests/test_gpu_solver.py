@@ -214,3 +214,41 @@ def test_autotuner_picks_a_timed_variant_and_keeps_parity(gpu_lib, oracle32, ora
     _, mo = masked_max_error(oracle32, fo.astype(np.float32), T, M, sp)
     _, mg = masked_max_error(oracle32, fg, T, M, sp)
     assert abs(mg - mo) <= MAX_ERR_TOL
+
+
+@pytest.mark.parametrize("variant", ["0", "1"])
+@pytest.mark.parametrize("precision", ["32", "64"])
+def test_greedy_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant, precision):
+    """The fused greedy pass (1) and the step + evaluation kernels (0), FP32
+    and FP64 stencils, each to the parity bar of test_solver_parity."""
+    monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", variant)
+    monkeypatch.setenv("DISPTOOLS_STENCIL_PRECISION", precision)
+    T, M, sp, it_max = CASES["sphere24_aniso"]()
+    print("greedy variant", variant, "precision", precision, T.shape, flush=True)
+    fo, io = oracle64.solve("greedy", T.astype(np.float64), M, sp, it_max=it_max)
+    fo = fo.astype(np.float32)
+    fg, ig, trace = gpu_lib.solve_host("greedy", T, M, sp, it_max=it_max, trace_cap=4096)
+    assert ig["reason"] == io["reason"]
+    _, mo = masked_max_error(oracle32, fo, T, M, sp)
+    _, mg = masked_max_error(oracle32, fg, T, M, sp)
+    assert abs(mg - mo) <= MAX_ERR_TOL
+    assert abs(ig["iterations"] - io["iterations"]) <= max(3, 0.1 * io["iterations"])
+    k = min(5, len(trace), len(io["trace"]))
+    np.testing.assert_allclose(trace[:k, 1], io["trace"][:k, 1], rtol=1e-4)
+
+
+def test_greedy_first_step_matches_between_variants(gpu_lib, monkeypatch):
+    """One greedy iteration: the fused pass forms the same trial as the step
+    kernel (the error maps it steps along agree to ~1 ulp of J)."""
+    from disptools_amd.workloads import volume_case
+    T, M = volume_case((37, 29, 23), 3)
+    sp = (1.0, 1.2, 0.9)
+    out = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", v)
+        print("greedy it_max=1 variant", v, flush=True)
+        out[v] = gpu_lib.solve_host("greedy", T, M, sp, it_max=1)
+    (f0, i0, _), (f1, i1, _) = out["0"], out["1"]
+    assert i0["reason"] == i1["reason"] and i0["iterations"] == i1["iterations"]
+    np.testing.assert_allclose(f1, f0, rtol=0, atol=2e-6)
+    assert i1["final_error"] == pytest.approx(i0["final_error"], rel=1e-5)
