@@ -12,7 +12,13 @@ roofline: the fused gradient pass (k_grad_stream).  Algorithmic bytes per
 evaluated pass = 12 B per field volume read or written (the controller counts
 them: 1 read + 1 speculative write on an accepted-path pass, 2 reads + 2
 writes after a rejection) + 5 B of target and mask, per voxel; achieved =
-those bytes / (HIP-event time of the solves on their stream / pass launches).
+those bytes / average launch duration, the device time of the launches (HIP
+events on the solve's stream around each chunk of back-to-back launches) over
+their count.
+
+--algorithm greedy times greedy solves of the same volume instead (the
+fused greedy pass, k_greedy_stream: 12 B per field volume read or written,
+4 B per error map read or written, 5 B of target and mask, per voxel).
 
 cpu_baseline: the reference C library built from its sources (oracle/_ref,
 kind "reference"; the oracle restatement, kind "port", if that build is
@@ -35,10 +41,11 @@ sys.path.insert(0, ROOT)
 METRIC = "voxel-iters/s at 128³ + p50/p99 solve latency @ fixed QPS vs plain GPU solve"
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_grad_stream launch of the default solve (128^3, it_max 2000,
-# default pass variant), from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
-# runs of this script, scaled by what the counters read for a 1 GiB copy
-# (--mode copy): read bytes = 2.0 x FETCH_SIZE, written = 1.0 x WRITE_SIZE.
-MEASURED_TRAFFIC_128 = 68.00e6
+# default pass variant: 64 x 12 column tiles), from rocprofv3 --pmc FETCH_SIZE
+# and --pmc WRITE_SIZE runs of this script, scaled by what the counters read
+# for a 1 GiB copy (--mode copy): read bytes = 2.0 x FETCH_SIZE, written =
+# 1.0 x WRITE_SIZE; the mean over all 44 launches of a solve.
+MEASURED_TRAFFIC_128 = 82.84e6
 
 
 def parse():
@@ -51,6 +58,8 @@ def parse():
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy"), default="solve")
+    ap.add_argument("--algorithm", choices=("gradient", "greedy"), default="gradient",
+                    help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
                     help="volumes of the batched-vs-plain comparison (0 to skip it)")
     ap.add_argument("--serve", type=int, default=0,
@@ -76,7 +85,7 @@ def copy_calibration():
                       "bytes_written_per_copy": 4 * n}))
 
 
-def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50):
+def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50, algorithm="gradient"):
     """Solves of the same inputs on the host, repeated until about
     `min_seconds` of CPU work (bounded by `max_repeats`)."""
     from oracle.oracle import Oracle, Reference, reference_available
@@ -91,17 +100,17 @@ def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50):
     while reps < max_repeats and (reps == 0 or dt < min_seconds):
         t0 = time.perf_counter()
         if ref is not None:
-            _, info = ref.solve("gradient", T, M, sp, it_max=iters, capture=True)
+            _, info = ref.solve(algorithm, T, M, sp, it_max=iters, capture=True)
             it = max(len(info["lines"]) - 1, 0)
         else:
-            _, info = orc.solve("gradient", T, M, sp, it_max=iters)
+            _, info = orc.solve(algorithm, T, M, sp, it_max=iters)
             it = max(len(info["trace"]) - 1, 0)
         dt += time.perf_counter() - t0
         done += it
         reps += 1
     return {"value": n * done / dt, "unit": "voxel-iters/s", "cores": threads, "kind": kind,
-            "sample": "%d gradient solve(s) of the same 128^3 inputs (it_max=%d), %d iterations "
-                      "in %.1f s" % (reps, iters, done, dt)}
+            "sample": "%d %s solve(s) of the same 128^3 inputs (it_max=%d), %d iterations "
+                      "in %.1f s" % (reps, algorithm, iters, done, dt)}
 
 
 def _iterations(info):
@@ -361,7 +370,7 @@ def main():
 
     def step():
         d_F.zero_()
-        info, _ = native.solve_device("gradient", nx, ny, nz, sp, d_T.data_ptr(), d_M.data_ptr(),
+        info, _ = native.solve_device(args.algorithm, nx, ny, nz, sp, d_T.data_ptr(), d_M.data_ptr(),
                                       d_F.data_ptr(), stream=sh, it_max=args.it_max)
         return info
 
@@ -390,7 +399,13 @@ def main():
     evals = sum(i["evaluations"] for i in infos)
     field_io = sum(i["field_io"] for i in infos)
     alg_bytes = n * (12.0 * field_io + 5.0 * evals)
-    avg_launch_s = (gpu_ms / 1e3) / launches
+    if args.algorithm == "greedy":
+        # error maps: every pass writes one, every pass but the initial
+        # evaluation also reads the previous one
+        alg_bytes += n * (4.0 * evals + 4.0 * (evals - len(infos)))
+    # device time of the pass launches (HIP events recorded by the library on
+    # the solve's stream around each chunk of back-to-back launches)
+    avg_launch_s = sum(i["kernel_seconds"] for i in infos) / launches
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
     batch = batch_bench(args.batch, args.it_max, args.graphs) if args.batch > 0 else None
@@ -398,9 +413,11 @@ def main():
                if args.serve > 0 else None)
 
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    cpu = cpu_baseline(T, M, sp, args.cpu_iters, threads) if args.cpu_iters > 0 else None
+    cpu = (cpu_baseline(T, M, sp, args.cpu_iters, threads, algorithm=args.algorithm)
+           if args.cpu_iters > 0 else None)
 
-    default_workload = args.it_max == 2000 and "DISPTOOLS_GRAD_VARIANT" not in os.environ
+    default_workload = (args.it_max == 2000 and args.algorithm == "gradient" and
+                        "DISPTOOLS_GRAD_VARIANT" not in os.environ)
     last = infos[-1]
     out = {
         "metric": METRIC,
@@ -415,13 +432,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "gradient solve, 128x128x128, spacing (1,1,1.5), smoothed log-normal "
-                               "target, ellipsoid mask, zero init, it_max=%d" % args.it_max,
+        "config": {"workload": "%s solve, 128x128x128, spacing (1,1,1.5), smoothed log-normal "
+                               "target, ellipsoid mask, zero init, it_max=%d" % (args.algorithm, args.it_max),
                    "voxels": n, "it_max": args.it_max},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": MEASURED_TRAFFIC_128 if default_workload else None,
-                     "kernel": "k_grad_stream", "avg_launch_us": avg_launch_s * 1e6,
+                     "kernel": "k_grad_stream" if args.algorithm == "gradient" else "k_greedy_stream",
+                     "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes / launches},
         "cpu_baseline": cpu,
         "batch": batch,

@@ -144,10 +144,11 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
     }
     const bool col_in = ins[0];
     const ColCoef K = column_coefficients(P, gxs[0], gys[0]);
-    // a block whose region meets the volume's x / y outer face reads the
-    // output buffer there (the face keeps its contents)
-    const bool xy_face = uni((int)(tc.x0 - 2 <= 0 || tc.x0 + TX + 1 >= nx - 1 || tc.y0 - 2 <= 0 ||
-                                   tc.y0 + TY + 1 >= ny - 1)) != 0;
+    // region points on the volume's x / y outer face keep the output
+    // buffer's contents in a step (the loads below skip the other lanes)
+    bool xyf[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) xyf[k] = ins[k] && !ints[k];
 
     auto zsoff = [&](int gz) -> unsigned { return (unsigned)min(max(gz, 0), nz - 1) * plane; };
 
@@ -158,24 +159,21 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
     auto fetch_e = [&](int gz, ERaw &r) {
         const unsigned so = 4u * zsoff(gz);
         r.e[0] = bload(r_ecur, 4u * vs[0], so);
-        r.e[1] = bload(r_ecur, 4u * vs[1], so);
-        r.e[2] = bload(r_ecur, 4u * v3, so);
+        r.e[1] = has1 ? bload(r_ecur, 4u * vs[1], so) : 0.0f;
+        r.e[2] = has3 ? bload(r_ecur, 4u * v3, so) : 0.0f;
     };
     auto fetch_u = [&](int gz, URaw &r) {
         const unsigned so = 4u * zsoff(gz);
+        const bool zface = gz <= 0 || gz >= nz - 1;
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < 2; ++k) {
+            const bool load = k == 0 || has1;
+            const bool face = step_on && ins[k] && (zface || xyf[k]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                r.b[k][c] = bload(r_base, 4u * vs[k], so + (unsigned)c * cn4);
-                r.g[k][c] = 0.0f;
+                r.b[k][c] = load ? bload(r_base, 4u * vs[k], so + (unsigned)c * cn4) : 0.0f;
+                r.g[k][c] = face ? bload(r_new, 4u * vs[k], so + (unsigned)c * cn4) : 0.0f;
             }
-        const bool zface = gz <= 0 || gz >= nz - 1;
-        if (step_on && (xy_face || zface)) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) r.g[k][c] = bload(r_new, 4u * vs[k], so + (unsigned)c * cn4);
         }
     };
     struct TM { float t; unsigned m; };

@@ -480,12 +480,25 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     hipEvent_t ev[2];
     DT_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     DT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    // timing events around each chunk's back-to-back pass launches: the
+    // device time of the passes, without the host turnaround between chunks
+    hipEvent_t kev[2][2] = {};
+    for (auto &pair : kev)
+        for (auto &e : pair) DT_HIP(hipEventCreate(&e));
+    bool pending[2] = {false, false};
+    double kernel_ms = 0.0;
+    auto account = [&](int s) {
+        float ms = 0.0f;
+        if (pending[s] && hipEventElapsedTime(&ms, kev[s][0], kev[s][1]) == hipSuccess) kernel_ms += ms;
+        pending[s] = false;
+    };
 
     // passes needed in the worst case before the controller stops the solve
     const long long pass_cap = max_passes(*p);
     long long launched = 0;
     int rc = 0;
     auto enqueue = [&](int k, int slot) -> int {
+        DT_HIP(hipEventRecord(kev[slot][0], st));
         for (int i = 0; i < k; ++i) {
             if (algorithm == DT_GRADIENT) {
                 DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, st));
@@ -494,6 +507,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
             }
         }
         launched += k;
+        DT_HIP(hipEventRecord(kev[slot][1], st));
+        pending[slot] = true;
         DT_HIP(hipMemcpyAsync(&A->host_ctrl[slot], h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
         DT_HIP(hipEventRecord(ev[slot], st));
         return 0;
@@ -508,6 +523,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     bool done = false;
     while (!rc) {
         if (hipEventSynchronize(ev[slot]) != hipSuccess) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
+        account(slot);
         if (A->host_ctrl[slot].phase == PH_DONE) { done = true; break; }
         if (launched > pass_cap) {
             set_error(__LINE__, __FILE__, __func__, "solver did not terminate (check alpha > 1 and beta < 1)");
@@ -518,8 +534,11 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         slot ^= 1;
     }
     hipError_t se = hipStreamSynchronize(st);
+    if (se == hipSuccess) { account(0); account(1); }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
+    for (auto &pair : kev)
+        for (auto &e : pair) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
     (void)done;
@@ -546,6 +565,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         info->trace_len = fin.trace_len;
         info->field_io = fin.field_io;
         info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        info->kernel_seconds = kernel_ms * 1e-3;
     }
     return 0;
 }

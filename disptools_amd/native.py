@@ -79,6 +79,7 @@ class SolveInfo(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("trace_len", ctypes.c_longlong),
         ("field_io", ctypes.c_longlong),
+        ("kernel_seconds", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -120,6 +120,9 @@ typedef struct dt_solve_info {
     long long trace_len;     /* log lines produced (it, E, max, eta each)   */
     long long field_io;      /* [3][nz][ny][nx] field volumes read+written by
                                 the evaluated passes (algorithmic traffic)  */
+    double kernel_seconds;   /* device time of the pass launches (events
+                                around each chunk of back-to-back launches;
+                                0 in engine results)                        */
 } dt_solve_info;
 
 /* Jacobian of a device-resident field (device pointers). Returns 0 or -1. */
