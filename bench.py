@@ -18,7 +18,9 @@ their count.
 
 --algorithm greedy times greedy solves of the same volume instead (the
 fused greedy pass, k_greedy_stream: 12 B per field volume read or written,
-4 B per error map read or written, 5 B of target and mask, per voxel).
+4 B per error map read or written, 5 B of target and mask, per voxel);
+--algorithm matching times the matching algorithm (k_match_pass: 44 B/voxel
+per cost pass, 28 B/voxel per gradient pass).
 
 cpu_baseline: the reference C library built from its sources (oracle/_ref,
 kind "reference"; the oracle restatement, kind "port", if that build is
@@ -58,7 +60,7 @@ def parse():
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy"), default="solve")
-    ap.add_argument("--algorithm", choices=("gradient", "greedy"), default="gradient",
+    ap.add_argument("--algorithm", choices=("gradient", "greedy", "matching"), default="gradient",
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
                     help="volumes of the batched-vs-plain comparison (0 to skip it)")
@@ -100,8 +102,15 @@ def cpu_baseline(T, M, sp, iters, threads, min_seconds=10.0, max_repeats=50, alg
     while reps < max_repeats and (reps == 0 or dt < min_seconds):
         t0 = time.perf_counter()
         if ref is not None:
-            _, info = ref.solve(algorithm, T, M, sp, it_max=iters, capture=True)
-            it = max(len(info["lines"]) - 1, 0)
+            if algorithm == "matching":
+                _, info = ref.matching(T, M, sp, it_max=iters)
+                it = len(info["trace"])
+            else:
+                _, info = ref.solve(algorithm, T, M, sp, it_max=iters, capture=True)
+                it = max(len(info["lines"]) - 1, 0)
+        elif algorithm == "matching":
+            _, info = orc.matching(T, sp, it_max=iters)
+            it = info["iterations"]
         else:
             _, info = orc.solve(algorithm, T, M, sp, it_max=iters)
             it = max(len(info["trace"]) - 1, 0)
@@ -370,6 +379,10 @@ def main():
 
     def step():
         d_F.zero_()
+        if args.algorithm == "matching":
+            info, _ = native.matching_device(nx, ny, nz, sp, d_T.data_ptr(), d_F.data_ptr(), stream=sh,
+                                             it_max=args.it_max)
+            return info
         info, _ = native.solve_device(args.algorithm, nx, ny, nz, sp, d_T.data_ptr(), d_M.data_ptr(),
                                       d_F.data_ptr(), stream=sh, it_max=args.it_max)
         return info
@@ -392,17 +405,27 @@ def main():
 
     # iterations performed: the counter ends at it_max + 1 after a full run, and
     # at the iteration whose trial triggered the stop otherwise
-    iters = sum(i["iterations"] - (1 if i["reason"] == 6 else 0) for i in infos)
+    launches = sum(i["launches"] for i in infos)
+    if args.algorithm == "matching":
+        # accepted iterations; a cost pass reads the accepted positions, the
+        # gradient and the target and writes the trial and its cell weights
+        # (44 B/voxel; the initial evaluation 20 B), a gradient pass reads the
+        # positions and weights and writes the gradient (28 B/voxel)
+        iters = sum(i["iterations"] for i in infos)
+        costs = sum(i["cost_passes"] for i in infos)
+        grads = sum(i["gradient_passes"] for i in infos)
+        alg_bytes = n * (44.0 * (costs - len(infos)) + 20.0 * len(infos) + 28.0 * grads)
+    else:
+        iters = sum(i["iterations"] - (1 if i["reason"] == 6 else 0) for i in infos)
+        evals = sum(i["evaluations"] for i in infos)
+        field_io = sum(i["field_io"] for i in infos)
+        alg_bytes = n * (12.0 * field_io + 5.0 * evals)
+        if args.algorithm == "greedy":
+            # error maps: every pass writes one, every pass but the initial
+            # evaluation also reads the previous one
+            alg_bytes += n * (4.0 * evals + 4.0 * (evals - len(infos)))
     voxel_iters = n * iters
     value = voxel_iters / wall
-    launches = sum(i["launches"] for i in infos)
-    evals = sum(i["evaluations"] for i in infos)
-    field_io = sum(i["field_io"] for i in infos)
-    alg_bytes = n * (12.0 * field_io + 5.0 * evals)
-    if args.algorithm == "greedy":
-        # error maps: every pass writes one, every pass but the initial
-        # evaluation also reads the previous one
-        alg_bytes += n * (4.0 * evals + 4.0 * (evals - len(infos)))
     # device time of the pass launches (HIP events recorded by the library on
     # the solve's stream around each chunk of back-to-back launches)
     avg_launch_s = sum(i["kernel_seconds"] for i in infos) / launches
@@ -413,7 +436,10 @@ def main():
                if args.serve > 0 else None)
 
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    cpu = (cpu_baseline(T, M, sp, args.cpu_iters, threads, algorithm=args.algorithm)
+    # the reference's matching is single-threaded and takes seconds per
+    # iteration at 128^3: its sample is a few iterations
+    cpu_iters = min(args.cpu_iters, 5) if args.algorithm == "matching" else args.cpu_iters
+    cpu = (cpu_baseline(T, M, sp, cpu_iters, threads, algorithm=args.algorithm)
            if args.cpu_iters > 0 else None)
 
     default_workload = (args.it_max == 2000 and args.algorithm == "gradient" and
@@ -438,16 +464,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": MEASURED_TRAFFIC_128 if default_workload else None,
-                     "kernel": "k_grad_stream" if args.algorithm == "gradient" else "k_greedy_stream",
+                     "kernel": {"gradient": "k_grad_stream", "greedy": "k_greedy_stream",
+                                "matching": "k_match_pass"}[args.algorithm],
                      "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes / launches},
         "cpu_baseline": cpu,
         "batch": batch,
         "serving": serving,
-        "solve": {"iterations": last["iterations"], "reason": native.REASONS[last["reason"]],
-                  "evaluations": last["evaluations"], "launches": last["launches"],
-                  "rejections": last["rejections"], "final_error": last["final_error"],
-                  "final_max_error": last["final_max_error"], "gpu_ms_per_step": gpu_ms / args.steps},
+        "solve": dict({k: v for k, v in last.items() if k not in ("seconds", "trace_len", "kernel_seconds")},
+                      reason=native.REASONS[last["reason"]], gpu_ms_per_step=gpu_ms / args.steps),
     }
     print(json.dumps(out))
 

@@ -39,6 +39,7 @@ HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_grad_pass.hip", "-ffp-contract=fast"),
                ("dt_split_pass.hip", "-ffp-contract=fast"),
                ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
+               ("dt_matching.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_pyramid.hip", "-ffp-contract=off"),
                ("dt_runtime.cpp", "-ffp-contract=off"),
                ("dt_engine.cpp", "-ffp-contract=off")]

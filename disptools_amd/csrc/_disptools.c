@@ -197,8 +197,8 @@ static PyObject *method_displacement(PyObject *self, PyObject *args)
     } else if (!strcmp(algorithm, ALGORITHM_GREEDY)) {
         fn = &generate_displacement_greedy_cuda;
     } else {
-        PyErr_Format(PyExc_ValueError, "Algorithm '%s' not supported on GPU", algorithm);
-        return NULL;
+        /* CPU-only in the reference (_disptools.c:368-373); here on the GPU */
+        fn = &volume_matching_3d;
     }
 
     const float *Jd = (const float *)PyArray_DATA(J);

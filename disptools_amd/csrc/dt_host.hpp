@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "disptools_hip.h"
@@ -17,6 +18,12 @@ namespace dt {
 
 // Sets the process-global error state (the contract of src/disptools.c:8-43).
 void set_error(int line, const char *file, const char *func, const char *msg);
+
+// Serialises the single-volume entry points (they share the device arena).
+std::mutex &solve_lock();
+// Device scratch of at least `bytes` on the current device (grow-only, reused
+// by later calls; valid until the next call under solve_lock()).
+int arena_device_buffer(size_t bytes, void **ptr);
 
 #define DT_FAIL(msg)                                         \
     do {                                                     \

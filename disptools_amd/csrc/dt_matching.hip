@@ -1,0 +1,787 @@
+// dt_matching.hip -- the `matching` algorithm (Karacali & Davatzikos volume
+// matching, src/VolumeMatching3D.c) on the GPU.
+//
+// Nodes (i, j, k) carry their deformed position in voxel units, F = y + 1 +
+// u_y / dy, G = x + 1 + u_x / dx, E = z + 1 + u_z / dz (volume_matching_3d,
+// :689-761).  Each cell (the cube of nodes i..i+1, j..j+1, k..k+1) has eight
+// corner Jacobians -- at corner q the determinant of the matrix whose rows are
+// F, G, E and whose columns are their forward differences along the cell
+// edges through q, in the order y, x, z (:119-145) -- and the target V of its
+// lowest node.  Cost (VolumeMatchingCost3D, :81-189):
+//     sum over cells of 0.5 ([V > 0] (S - 8 V)^2 + sum_q [J_q < delta] zeta (J_q - delta)^2),
+// S = sum of the corner Jacobians; max voxel error = max over cells of
+// |S - 8 V| / 8.  Gradient (VolumeMatchingGrads3D, :191-406): each cell gives
+// each of its nodes W_q times the cofactor of the matrix entry the node enters,
+// W_q = [V > 0] (S - 8 V) + [J_q < delta] zeta (J_q - delta), for the corner
+// q at the node and its three edge neighbours.
+//
+// One kernel, k_match_pass, runs either pass over z-chunked column tiles; the
+// control block says which.  A cost pass evaluates the trial X - alpha D (or
+// X itself for the initial evaluation), stores it (it becomes the accepted
+// point if the step is taken) with its cell weights [V > 0](S - 8 V), and
+// sums the cost in FP64 with the max voxel error; a gradient pass forms D at
+// the accepted point from those weights.  The last block of every pass folds
+// the per-tile partials in tile order and runs the step-length control of
+// VolumeMatching3D (:441-545): start at step 1; on the first iteration
+// double the step while the cost keeps falling and take the last improving
+// trial; later iterations take an improving step and grow it by 1.05;
+// otherwise halve it until the cost no longer rises or the step falls to
+// 1e-9.  The loop runs while iterations < it_max, step > 1e-9 and max voxel
+// error > epsilon, the error being the one of the last evaluated trial (as in
+// the reference, where it is a global set by every cost call).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "dt_host.hpp"
+
+namespace dt {
+namespace {
+
+constexpr int MX = 32, MY = 8, MNT = MX * MY;    // column tile
+constexpr int CPX = MX + 1, CPP = CPX * (MY + 1);  // cost pass: node plane with +1 halo
+constexpr int GPX = MX + 2, GPP = GPX * (MY + 2);  // gradient pass: node plane with a 1-ring
+constexpr int NGRING = GPP - MNT;                  // 2 MX + 2 MY + 4
+constexpr int NCRING = CPP - MNT;                  // MX + MY + 1
+static_assert(NGRING <= MNT && NCRING <= MNT, "one extra point per thread at most");
+
+enum { MPH_INIT = 0, MPH_GRAD = 1, MPH_COST = 2, MPH_DONE = 3 };
+enum { MM_FIRST = 0, MM_GROW = 1, MM_SHRINK = 2 };
+
+struct MCtrl {
+    int phase, active, mode;
+    int xb, tb, pb;            // accepted triple, trial triple, previous trial (growth)
+    float alpha, m;            // step; max voxel error of the last cost pass
+    double H, Hmark;           // cost of the accepted point; line-search reference
+    long long it, cost_passes, grad_passes, trace_len;
+    int reason;
+};
+
+struct MInst {
+    int nx, ny, nz, tz, ntx, nty, ntz;
+    float delta, zeta, epsilon;
+    long long it_max;
+    long long n;
+    float *tri[3];             // triple t: F, G, E at tri[t] + {0, n, 2n}
+    float *w[3];               // cell weights of triple t, node-indexed
+    float *D;                  // gradient, F, G, E at D + {0, n, 2n}
+    const float *V;            // target, node-indexed
+    double *partials;          // 2 per tile
+    unsigned *counter;
+    MCtrl *ctrl;
+    double *trace;
+    long long trace_cap;
+};
+
+// Matrix columns: 0 = difference along y, 1 = along x, 2 = along z; corner
+// bits: 1 = +x, 2 = +y, 4 = +z.
+constexpr int COL_BIT[3] = {2, 1, 4};
+// the reference sums the corner terms in the order of its names J_abc,
+// a = z, b = x, c = y (VolumeMatching3D.c:151-160)
+constexpr int REF_ORDER[8] = {0, 2, 1, 3, 4, 6, 5, 7};
+
+__device__ __forceinline__ void corner_matrix(const float (&v)[3][8], int q, float (&m)[3][3])
+{
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m[r][c] = v[r][q | COL_BIT[c]] - v[r][q & ~COL_BIT[c]];
+}
+
+__device__ __forceinline__ float det3(const float (&m)[3][3])
+{
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+
+// cofactor (r, c) of m
+__device__ __forceinline__ float cof(const float (&m)[3][3], int r, int c)
+{
+    const int r1 = r == 0 ? 1 : 0, r2 = r == 2 ? 1 : 2;
+    const int c1 = c == 0 ? 1 : 0, c2 = c == 2 ? 1 : 2;
+    const float minor = m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1];
+    return ((r + c) & 1) ? -minor : minor;
+}
+
+// x - a d, rounded as the reference's C (no FMA; this file is compiled with
+// -ffp-contract=fast-honor-pragmas)
+__device__ __forceinline__ float step_value(float x, float a, float d)
+{
+#pragma clang fp contract(off)
+    return x - a * d;
+}
+
+__device__ __forceinline__ long long node_index(int nx, int ny, int i, int j, int k)
+{
+    return ((long long)k * ny + j) * nx + i;
+}
+
+// ---------------------------------------------------------------------------
+// Passes
+// ---------------------------------------------------------------------------
+
+struct CostSmem {
+    float s[3][3][CPP];        // trial node planes, ring of three
+};
+struct GradSmem {
+    float x[4][3][GPP];        // accepted node planes, ring of four
+    float w[3][CPP];           // cell weights, ring of three
+};
+union PassSmem {
+    CostSmem c;
+    GradSmem g;
+};
+
+__device__ void cost_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0, int z1, PassSmem &S,
+                          double &Hs, float &mx)
+{
+    const int nx = I.nx, ny = I.ny, nz = I.nz;
+    const long long n = I.n;
+    const bool base = C.phase == MPH_INIT;
+    const float *X = I.tri[C.xb];
+    float *T = base ? nullptr : I.tri[C.tb];
+    float *W = I.w[base ? C.xb : C.tb];
+    const float *D = I.D;
+    const float alpha = C.alpha;
+    const int tid = threadIdx.x;
+    const int lx = tid % MX, ly = tid / MX;
+    // node points: own, and the +1 halo row / column
+    int px[2], py[2];
+    px[0] = lx; py[0] = ly;
+    const bool has1 = tid < NCRING;
+    if (tid < MX + 1) { px[1] = tid; py[1] = MY; }
+    else { px[1] = MX; py[1] = tid - (MX + 1); }
+    bool ok[2];
+    long long nid[2];
+    int li[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int gx = x0 + px[k], gy = y0 + py[k];
+        ok[k] = (k == 0 || has1) && gx < nx && gy < ny;
+        nid[k] = ok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
+        li[k] = py[k] * CPX + px[k];
+    }
+    const int gx = x0 + lx, gy = y0 + ly;
+    const bool cell_ok = gx < nx - 1 && gy < ny - 1;
+    const long long plane = (long long)nx * ny;
+    const int pend = min(z1, nz - 1);
+    for (int p = z0; p <= pend; ++p) {
+        float(*dst)[CPP] = S.c.s[p % 3];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 || has1)) continue;
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            if (ok[k]) {
+                const long long id = nid[k] + p * plane;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    v[c] = base ? X[c * n + id] : step_value(X[c * n + id], alpha, D[c * n + id]);
+                if (k == 0 && T && p < z1) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) T[c * n + id] = v[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dst[c][li[k]] = v[c];
+        }
+        __syncthreads();
+        const int kc = p - 1;
+        if (kc >= z0 && cell_ok) {
+            const float(*lo)[CPP] = S.c.s[kc % 3];
+            const float(*hi)[CPP] = S.c.s[p % 3];
+            float v[3][8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = (ly + ((q >> 1) & 1)) * CPX + lx + (q & 1);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[c][q] = (q & 4) ? hi[c][idx] : lo[c][idx];
+            }
+            float J[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                float m[3][3];
+                corner_matrix(v, q, m);
+                J[q] = det3(m);
+            }
+            float Ssum = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) Ssum += J[REF_ORDER[t]];
+            const long long cid = node_index(nx, ny, gx, gy, kc);
+            const float cv = I.V[cid];
+            const float r = Ssum - 8.0f * cv;
+            float cres = cv > 0.0f ? r * r : 0.0f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const float Jq = J[REF_ORDER[t]];
+                if (Jq < I.delta) cres += I.zeta * (Jq - I.delta) * (Jq - I.delta);
+            }
+            Hs += 0.5 * (double)cres;
+            mx = fmaxf(mx, fabsf(r) * 0.125f);
+            W[cid] = cv > 0.0f ? (float)((double)Ssum - 8.0 * (double)cv) : 0.0f;
+        }
+    }
+}
+
+__device__ void grad_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0, int z1, PassSmem &S)
+{
+    const int nx = I.nx, ny = I.ny, nz = I.nz;
+    const long long n = I.n;
+    const float *X = I.tri[C.xb];
+    const float *Wc = I.w[C.xb];
+    float *D = I.D;
+    const float delta = I.delta, zeta = I.zeta;
+    const int tid = threadIdx.x;
+    const int lx = tid % MX, ly = tid / MX;
+    const long long plane = (long long)nx * ny;
+    // node points in ring coordinates [0, MX+2) x [0, MY+2): own (lx+1, ly+1),
+    // and the ring around the tile
+    int px[2], py[2];
+    px[0] = lx + 1; py[0] = ly + 1;
+    const bool has1 = tid < NGRING;
+    {
+        const int r = tid;
+        if (r < GPX) { px[1] = r; py[1] = 0; }
+        else if (r < 2 * GPX) { px[1] = r - GPX; py[1] = MY + 1; }
+        else { const int q = r - 2 * GPX; px[1] = (q & 1) ? MX + 1 : 0; py[1] = 1 + (q >> 1); }
+    }
+    bool ok[2];
+    long long nid[2];
+    int li[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int gx = x0 - 1 + px[k], gy = y0 - 1 + py[k];
+        ok[k] = (k == 0 || has1) && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        nid[k] = ok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
+        li[k] = py[k] * GPX + px[k];
+    }
+    // cell points in cell coordinates [0, MX+1) x [0, MY+1) (cell x0-1+cx):
+    // own (lx+1, ly+1), and the low row / column
+    int cx[2], cy[2];
+    cx[0] = lx + 1; cy[0] = ly + 1;
+    const bool hasc = tid < NCRING;
+    if (tid < MX + 1) { cx[1] = tid; cy[1] = 0; }
+    else { cx[1] = 0; cy[1] = tid - MX; }
+    bool cok[2];
+    long long cid[2];
+    int ci[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int gx = x0 - 1 + cx[k], gy = y0 - 1 + cy[k];
+        cok[k] = (k == 0 || hasc) && gx >= 0 && gx < nx - 1 && gy >= 0 && gy < ny - 1;
+        cid[k] = cok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
+        ci[k] = cy[k] * CPX + cx[k];
+    }
+    const int gx = x0 + lx, gy = y0 + ly;
+    const bool own = gx < nx && gy < ny;
+
+    for (int p = z0 - 1; p <= z1; ++p) {
+        // node plane p and cell plane p - 1 join the rings
+        const bool zin = p >= 0 && p < nz;
+        float(*xd)[GPP] = S.g.x[(p + 4) % 4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 || has1)) continue;
+            const long long id = nid[k] + (long long)min(max(p, 0), nz - 1) * plane;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xd[c][li[k]] = (zin && ok[k]) ? X[c * n + id] : 0.0f;
+        }
+        const int pc = p - 1;
+        const bool cin = pc >= 0 && pc < nz - 1;
+        float *wd = S.g.w[(pc + 3) % 3];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 || hasc)) continue;
+            wd[ci[k]] = (cin && cok[k]) ? Wc[cid[k] + (long long)min(max(pc, 0), nz - 1) * plane] : 0.0f;
+        }
+        __syncthreads();
+        const int kk = p - 1;   // node plane evaluated at this step
+        if (kk < z0 || kk >= z1 || !own) continue;
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        // cells (i-1+ax, j-1+ay, k-1+az), in cell order
+#pragma unroll
+        for (int az = 0; az < 2; ++az)
+#pragma unroll
+            for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+                for (int ax = 0; ax < 2; ++ax) {
+                    const int ck = kk - 1 + az, cj = gy - 1 + ay, cc = gx - 1 + ax;
+                    if (ck < 0 || ck >= nz - 1 || cj < 0 || cj >= ny - 1 || cc < 0 || cc >= nx - 1) continue;
+                    const float wcell = S.g.w[(ck + 3) % 3][(ly + ay) * CPX + lx + ax];
+                    float v[3][8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const float(*xs)[GPP] = S.g.x[(ck + ((q >> 2) & 1) + 4) % 4];
+                        const int idx = (ly + ay + ((q >> 1) & 1)) * GPX + lx + ax + (q & 1);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) v[c][q] = xs[c][idx];
+                    }
+                    // the node is corner p of this cell
+                    const int pq = (1 - ax) | ((1 - ay) << 1) | ((1 - az) << 2);
+                    float m0[3][3];
+                    corner_matrix(v, pq, m0);
+                    const float J0 = det3(m0);
+                    const float W0 = wcell + (J0 < delta ? zeta * (J0 - delta) : 0.0f);
+                    float part[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int col = 0; col < 3; ++col) {
+                        const int b = COL_BIT[col];
+                        const int qn = pq ^ b;
+                        float mn[3][3];
+                        corner_matrix(v, qn, mn);
+                        const float Jn = det3(mn);
+                        const float Wn = wcell + (Jn < delta ? zeta * (Jn - delta) : 0.0f);
+                        const float sgn = (pq & b) ? 1.0f : -1.0f;
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) part[r] += sgn * (W0 * cof(m0, r, col) + Wn * cof(mn, r, col));
+                    }
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) acc[r] += part[r];
+                }
+        const long long id = node_index(nx, ny, gx, gy, kk);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) D[r * n + id] = acc[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Control (one thread of the last block)
+// ---------------------------------------------------------------------------
+
+__device__ void m_finish(const MInst &I, MCtrl *C)
+{
+    C->phase = MPH_DONE;
+    C->active = 0;
+    C->reason = C->it >= I.it_max ? DT_STOP_MAX_ITERATIONS
+                                  : (!((double)C->alpha > 1e-9) ? DT_STOP_ETA_SMALL : DT_STOP_TOLERANCE);
+}
+
+__device__ void m_loop_check(const MInst &I, MCtrl *C)
+{
+    if (C->it < I.it_max && (double)C->alpha > 1e-9 && C->m > I.epsilon) C->phase = MPH_GRAD;
+    else m_finish(I, C);
+}
+
+__device__ void m_accept(const MInst &I, MCtrl *C, int t, double Hn)
+{
+    if (!((double)C->alpha > 1e-9)) { m_finish(I, C); return; }
+    C->xb = t;
+    C->H = Hn;
+    C->it++;
+    if (I.trace && C->trace_len < I.trace_cap) {
+        double *row = I.trace + 4 * C->trace_len;
+        row[0] = (double)C->it; row[1] = C->H; row[2] = C->m; row[3] = C->alpha;
+    }
+    C->trace_len++;
+    m_loop_check(I, C);
+}
+
+__device__ __forceinline__ int third(int a, int b)
+{
+    for (int t = 0; t < 3; ++t)
+        if (t != a && t != b) return t;
+    return 0;
+}
+
+__device__ void match_controller(const MInst &I, MCtrl *C, double nH, float nm)
+{
+    if (C->phase == MPH_GRAD) {
+        C->grad_passes++;
+        C->Hmark = C->H;
+        C->mode = MM_FIRST;
+        C->tb = third(C->xb, C->xb == 0 ? 1 : 0);
+        C->phase = MPH_COST;
+        return;
+    }
+    C->cost_passes++;
+    C->m = nm;
+    if (C->phase == MPH_INIT) {
+        C->H = nH;
+        m_loop_check(I, C);
+        return;
+    }
+    switch (C->mode) {
+    case MM_FIRST:
+        if (nH < C->Hmark) {
+            if (C->it == 0) {
+                // grow the step while the cost keeps falling (:459-471)
+                C->Hmark = nH;
+                C->alpha = C->alpha * 2.0f;
+                C->pb = C->tb;
+                C->tb = third(C->xb, C->pb);
+                C->mode = MM_GROW;
+                return;
+            }
+            C->alpha = (float)((double)C->alpha * 1.05);
+            m_accept(I, C, C->tb, nH);
+            return;
+        }
+        if (nH > C->Hmark && (double)C->alpha > 1e-9) {
+            C->alpha = C->alpha * 0.5f;
+            C->mode = MM_SHRINK;
+            return;
+        }
+        m_accept(I, C, C->tb, nH);
+        return;
+    case MM_GROW:
+        if (nH < C->Hmark) {
+            C->Hmark = nH;
+            C->alpha = C->alpha * 2.0f;
+            const int t = C->pb;
+            C->pb = C->tb;
+            C->tb = t;
+            return;
+        }
+        // the previous trial was the last improving one (:472-480)
+        C->alpha = C->alpha * 0.5f;
+        m_accept(I, C, C->pb, C->Hmark);
+        return;
+    default:   // MM_SHRINK (:487-499)
+        if (nH > C->Hmark && (double)C->alpha > 1e-9) {
+            C->alpha = C->alpha * 0.5f;
+            return;
+        }
+        m_accept(I, C, C->tb, nH);
+        return;
+    }
+}
+
+__global__ void __launch_bounds__(MNT)
+k_match_pass(const MInst *__restrict__ pinst)
+{
+    __shared__ PassSmem S;
+    __shared__ double red[2 * (MNT / 64)];
+    __shared__ int last;
+    const MInst &I = *pinst;
+    MCtrl *C = I.ctrl;
+    if (!C->active) return;
+    const MCtrl ctl = *C;
+    const int t = blockIdx.x;
+    const int tx = t % I.ntx, ty = (t / I.ntx) % I.nty, tz = t / (I.ntx * I.nty);
+    const int x0 = tx * MX, y0 = ty * MY, z0 = tz * I.tz, z1 = min(z0 + I.tz, I.nz);
+    double Hs = 0.0;
+    float mx = 0.0f;
+    if (ctl.phase == MPH_GRAD) grad_pass(I, ctl, x0, y0, z0, z1, S);
+    else cost_pass(I, ctl, x0, y0, z0, z1, S, Hs, mx);
+
+    // block partials, then the last block folds all tiles in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        Hs += __shfl_xor(Hs, o, 64);
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { red[2 * wv] = Hs; red[2 * wv + 1] = (double)mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < MNT / 64; ++k) { a += red[2 * k]; b = fmax(b, red[2 * k + 1]); }
+        __hip_atomic_store(&I.partials[2 * t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&I.partials[2 * t + 1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned prev = __hip_atomic_fetch_add(I.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = prev + 1 == (unsigned)gridDim.x;
+    }
+    __syncthreads();
+    if (!last) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    double a = 0.0, b = 0.0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += MNT) {
+        a += __hip_atomic_load(&I.partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = fmax(b, __hip_atomic_load(&I.partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b = fmax(b, __shfl_xor(b, o, 64));
+    }
+    __syncthreads();
+    if (lane == 0) { red[2 * wv] = a; red[2 * wv + 1] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sa = 0.0, sb = 0.0;
+        for (int k = 0; k < MNT / 64; ++k) { sa += red[2 * k]; sb = fmax(sb, red[2 * k + 1]); }
+        match_controller(I, C, sa, (float)sb);
+        __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// field -> node positions (volume_matching_3d, :746-757)
+__global__ void __launch_bounds__(256)
+k_match_in(const float *__restrict__ field, float *__restrict__ X, int nx, int ny, int nz, float dx, float dy,
+           float dz)
+{
+    const long long n = (long long)nx * ny * nz;
+    for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
+        X[v] = (float)(j + 1) + field[n + v] / dy;
+        X[n + v] = (float)(i + 1) + field[v] / dx;
+        X[2 * n + v] = (float)(k + 1) + field[2 * n + v] / dz;
+    }
+}
+
+// node positions -> field (:769-781)
+__global__ void __launch_bounds__(256)
+k_match_out(const float *__restrict__ X, float *__restrict__ field, int nx, int ny, int nz, float dx, float dy,
+            float dz)
+{
+    const long long n = (long long)nx * ny * nz;
+    for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
+        field[v] = (X[n + v] - (float)(i + 1)) * dx;
+        field[n + v] = (X[v] - (float)(j + 1)) * dy;
+        field[2 * n + v] = (X[2 * n + v] - (float)(k + 1)) * dz;
+    }
+}
+
+unsigned grid_of(long long work)
+{
+    long long b = (work + 255) / 256;
+    return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
+}
+
+// z-chunk depth: about 8 blocks per CU in one round
+int match_tz(int nx, int ny, int nz)
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    const long long cols = (long long)((nx + MX - 1) / MX) * ((ny + MY - 1) / MY);
+    const long long want = 8ll * cus;
+    const long long chunks = std::max<long long>(1, std::min<long long>(nz, (want + cols - 1) / cols));
+    return std::max(2, (int)((nz + chunks - 1) / chunks));
+}
+
+struct MatchHost {
+    MCtrl *ctrl = nullptr;     // pinned, 2 slots
+};
+std::vector<MatchHost> g_mhost;
+
+}  // namespace
+
+// Matching solve on device buffers; `field` [3][nz][ny][nx] in/out.
+static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz, const float *d_target,
+                           float epsilon, float delta, float zeta, long long it_max, float *d_field,
+                           dt_matching_info *info, double *trace, long long trace_cap, hipStream_t st)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    const long long n = (long long)nx * ny * nz;
+    MInst h{};
+    h.nx = nx; h.ny = ny; h.nz = nz;
+    h.tz = match_tz(nx, ny, nz);
+    h.ntx = (nx + MX - 1) / MX;
+    h.nty = (ny + MY - 1) / MY;
+    h.ntz = (nz + h.tz - 1) / h.tz;
+    h.delta = delta; h.zeta = zeta; h.epsilon = epsilon;
+    h.it_max = it_max < 0 ? 0 : it_max;
+    h.n = n;
+    const int ntiles = h.ntx * h.nty * h.ntz;
+    if (trace_cap < 0 || !trace) trace_cap = 0;
+
+    size_t bytes = 0;
+    auto add = [&](size_t b) { bytes = align_up(bytes, 256) + b; };
+    for (int t = 0; t < 3; ++t) add(12 * (size_t)n);
+    for (int t = 0; t < 3; ++t) add(4 * (size_t)n);
+    add(12 * (size_t)n);
+    add(16 * (size_t)ntiles);
+    add(sizeof(MCtrl));
+    add(sizeof(unsigned));
+    add(sizeof(MInst));
+    add(32 * (size_t)std::max<long long>(trace_cap, 1));
+    void *base = nullptr;
+    if (arena_device_buffer(bytes + 4096, &base)) return -1;
+    Carver cv(base);
+    for (int t = 0; t < 3; ++t) h.tri[t] = cv.take<float>(3 * (size_t)n);
+    for (int t = 0; t < 3; ++t) h.w[t] = cv.take<float>((size_t)n);
+    h.D = cv.take<float>(3 * (size_t)n);
+    h.partials = cv.take<double>(2 * (size_t)ntiles);
+    h.ctrl = cv.take<MCtrl>(1);
+    h.counter = cv.take<unsigned>(1);
+    MInst *d_inst = cv.take<MInst>(1);
+    h.trace = trace_cap ? cv.take<double>(4 * (size_t)trace_cap) : nullptr;
+    h.trace_cap = trace_cap;
+    h.V = d_target;
+
+    int dev = 0;
+    DT_HIP(hipGetDevice(&dev));
+    if ((int)g_mhost.size() <= dev) g_mhost.resize(dev + 1);
+    MatchHost &mh = g_mhost[dev];
+    if (!mh.ctrl) DT_HIP(hipHostMalloc((void **)&mh.ctrl, 2 * sizeof(MCtrl), hipHostMallocDefault));
+
+    MCtrl c0{};
+    c0.phase = MPH_INIT;
+    c0.active = 1;
+    c0.xb = 0;
+    c0.alpha = 1.0f;
+    DT_HIP(hipMemcpyAsync(d_inst, &h, sizeof(MInst), hipMemcpyHostToDevice, st));
+    DT_HIP(hipMemcpyAsync(h.ctrl, &c0, sizeof(MCtrl), hipMemcpyHostToDevice, st));
+    DT_HIP(hipMemsetAsync(h.counter, 0, sizeof(unsigned), st));
+    hipLaunchKernelGGL(k_match_in, dim3(grid_of(n)), dim3(256), 0, st, d_field, h.tri[0], nx, ny, nz, dx, dy, dz);
+    DT_HIP(hipGetLastError());
+
+    hipEvent_t ev[2] = {nullptr, nullptr}, kev[2][2] = {};
+    for (auto &e : ev) DT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &pair : kev)
+        for (auto &e : pair) DT_HIP(hipEventCreate(&e));
+    bool pending[2] = {false, false};
+    double kernel_ms = 0.0;
+    auto account = [&](int s) {
+        float ms = 0.0f;
+        if (pending[s] && hipEventElapsedTime(&ms, kev[s][0], kev[s][1]) == hipSuccess) kernel_ms += ms;
+        pending[s] = false;
+    };
+    // every iteration takes one gradient pass and at most ~160 cost passes
+    // (the step doubles from 1 at most 128 times and halves at most 31 times
+    // before falling under 1e-9)
+    const long long pass_cap = 256 + 192 * (h.it_max + 1);
+    long long launched = 0;
+    const int chunk = 8;
+    auto enqueue = [&](int slot) -> int {
+        DT_HIP(hipEventRecord(kev[slot][0], st));
+        for (int i = 0; i < chunk; ++i) {
+            hipLaunchKernelGGL(k_match_pass, dim3(ntiles), dim3(MNT), 0, st, d_inst);
+            DT_HIP(hipGetLastError());
+        }
+        launched += chunk;
+        DT_HIP(hipEventRecord(kev[slot][1], st));
+        pending[slot] = true;
+        DT_HIP(hipMemcpyAsync(&mh.ctrl[slot], h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost, st));
+        DT_HIP(hipEventRecord(ev[slot], st));
+        return 0;
+    };
+    int rc = enqueue(0);
+    if (!rc) rc = enqueue(1);
+    int slot = 0;
+    while (!rc) {
+        if (hipEventSynchronize(ev[slot]) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "device error during the matching solve");
+            rc = -1;
+            break;
+        }
+        account(slot);
+        if (mh.ctrl[slot].phase == MPH_DONE) break;
+        if (launched > pass_cap) {
+            set_error(__LINE__, __FILE__, __func__, "matching solve did not terminate");
+            rc = -1;
+            break;
+        }
+        rc = enqueue(slot);
+        slot ^= 1;
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    if (se == hipSuccess) { account(0); account(1); }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &pair : kev)
+        for (auto &e : pair) (void)hipEventDestroy(e);
+    if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
+    if (rc) return -1;
+
+    MCtrl fin;
+    DT_HIP(hipMemcpy(&fin, h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(k_match_out, dim3(grid_of(n)), dim3(256), 0, st, h.tri[fin.xb], d_field, nx, ny, nz, dx, dy,
+                       dz);
+    DT_HIP(hipGetLastError());
+    if (trace && trace_cap > 0) {
+        const long long nl = std::min(fin.trace_len, trace_cap);
+        if (nl > 0) DT_HIP(hipMemcpyAsync(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    DT_HIP(hipStreamSynchronize(st));
+    if (info) {
+        info->iterations = fin.it;
+        info->reason = fin.reason;
+        info->cost_passes = fin.cost_passes;
+        info->gradient_passes = fin.grad_passes;
+        info->launches = launched;
+        info->final_cost = fin.H;
+        info->final_max_error = fin.m;
+        info->trace_len = fin.trace_len;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        info->kernel_seconds = kernel_ms * 1e-3;
+    }
+    return 0;
+}
+
+static int matching_validate(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float epsilon,
+                             float delta, float zeta)
+{
+    if (nx < 2 || ny < 2 || nz < 2) DT_FAIL("every image dimension must be at least 2 voxels");
+    if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) DT_FAIL("image too large");
+    if (!(dx > 0.0f) || !(dy > 0.0f) || !(dz > 0.0f)) DT_FAIL("spacing must be positive");
+    if (!(epsilon > 0.0f)) DT_FAIL("epsilon must be positive");
+    if (!(delta > 0.0f) || !(zeta > 0.0f)) DT_FAIL("delta and zeta must be positive");
+    return 0;
+}
+
+}  // namespace dt
+
+using namespace dt;
+
+extern "C" int dt_matching_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                                  const float *d_target, float epsilon, float delta, float zeta, long long it_max,
+                                  float *d_field, dt_matching_info *info, double *trace, long long trace_cap,
+                                  void *stream)
+{
+    std::lock_guard<std::mutex> g(solve_lock());
+    disptools_clear_error();
+    if (matching_validate(nx, ny, nz, dx, dy, dz, epsilon, delta, zeta)) return -1;
+    if (!d_target || !d_field) DT_FAIL("null buffer");
+    return matching_device((int)nx, (int)ny, (int)nz, dx, dy, dz, d_target, epsilon, delta, zeta, it_max, d_field,
+                           info, trace, trace_cap, (hipStream_t)stream);
+}
+
+extern "C" int dt_matching_host(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const float *target,
+                                float epsilon, float delta, float zeta, long long it_max, float *field,
+                                dt_matching_info *info, double *trace, long long trace_cap)
+{
+    std::lock_guard<std::mutex> g(solve_lock());
+    disptools_clear_error();
+    if (matching_validate(nx, ny, nz, dx, dy, dz, epsilon, delta, zeta)) return -1;
+    if (!target || !field) DT_FAIL("null buffer");
+    const size_t n = nx * ny * nz;
+    float *d = nullptr;
+    if (hipMalloc((void **)&d, 16 * n) != hipSuccess) {
+        (void)hipGetLastError();
+        DT_FAIL("cannot allocate device memory for the matching inputs");
+    }
+    float *dT = d, *dF = d + n;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipMemcpyAsync(dT, target, 4 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dF, field, 12 * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error(__LINE__, __FILE__, __func__, "cannot stage the matching inputs");
+        rc = -1;
+    }
+    if (!rc)
+        rc = matching_device((int)nx, (int)ny, (int)nz, dx, dy, dz, dT, epsilon, delta, zeta, it_max, dF, info,
+                             trace, trace_cap, st);
+    if (!rc && (hipMemcpyAsync(field, dF, 12 * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)) {
+        set_error(__LINE__, __FILE__, __func__, "cannot copy the matching result back");
+        rc = -1;
+    }
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(d);
+    return rc;
+}
+
+// The reference's matching entry point (src/headers/VolumeMatching3D.h:7-29,
+// src/VolumeMatching3D.c:689-796) on the current HIP device: host buffers,
+// the field in place.  Uses epsilon, delta, zeta and it_max; the other solver
+// parameters and the mask are accepted and unused, as in the reference.
+extern "C" void volume_matching_3d(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const float *J,
+                                   const bool *mask, float epsilon, float tolerance, float eta, float eta_max,
+                                   float alpha, float beta, float gamma, float delta, float zeta, float theta,
+                                   float iota, bool strict, size_t it_max, float *field)
+{
+    (void)mask; (void)tolerance; (void)eta; (void)eta_max; (void)alpha; (void)beta; (void)gamma;
+    (void)theta; (void)iota; (void)strict;
+    (void)dt_matching_host(nx, ny, nz, dx, dy, dz, J, epsilon, delta, zeta, (long long)it_max, field, nullptr,
+                           nullptr, 0);
+}

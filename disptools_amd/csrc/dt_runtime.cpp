@@ -97,6 +97,16 @@ static int arena_get(size_t bytes, Arena **out)
     return 0;
 }
 
+std::mutex &solve_lock() { return g_lock; }
+
+int arena_device_buffer(size_t bytes, void **ptr)
+{
+    Arena *A = nullptr;
+    if (arena_get(bytes, &A)) return -1;
+    *ptr = A->ptr;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Instance planning (shared with the batch engine)
 // ---------------------------------------------------------------------------

@@ -208,8 +208,12 @@ def _pyramid_displacement(J, M, guess, spacing, levels, it_max, parameters):
         T = J_levels[level].contiguous()
         Mb = (M_levels[level] != 0).to(torch.uint8).contiguous()
         field = field.contiguous()
-        native.solve_device(parameters["algorithm"], mx, my, mz, sp_levels[level], T.data_ptr(), Mb.data_ptr(),
-                            field.data_ptr(), stream=stream, it_max=it_max[k], **params)
+        if parameters["algorithm"] == "matching":
+            native.matching_device(mx, my, mz, sp_levels[level], T.data_ptr(), field.data_ptr(), stream=stream,
+                                   it_max=it_max[k], **params)
+        else:
+            native.solve_device(parameters["algorithm"], mx, my, mz, sp_levels[level], T.data_ptr(),
+                                Mb.data_ptr(), field.data_ptr(), stream=stream, it_max=it_max[k], **params)
         f_sp = sp_levels[level]
     return np.ascontiguousarray(np.moveaxis(field.cpu().numpy(), 0, -1))
 

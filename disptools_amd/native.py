@@ -114,6 +114,34 @@ lib.dt_solve_host.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp,
                               ctypes.POINTER(Params), _vp, ctypes.POINTER(SolveInfo), _vp,
                               ctypes.c_longlong]
 lib.dt_solve_host.restype = ctypes.c_int
+lib.volume_matching_3d.argtypes = _ref_solver
+lib.volume_matching_3d.restype = None
+
+
+class MatchingInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_longlong),
+        ("reason", ctypes.c_int),
+        ("cost_passes", ctypes.c_longlong),
+        ("gradient_passes", ctypes.c_longlong),
+        ("launches", ctypes.c_longlong),
+        ("final_cost", ctypes.c_double),
+        ("final_max_error", ctypes.c_double),
+        ("trace_len", ctypes.c_longlong),
+        ("seconds", ctypes.c_double),
+        ("kernel_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+lib.dt_matching_device.argtypes = [_sz, _sz, _sz, _f, _f, _f, _vp, _f, _f, _f, ctypes.c_longlong, _vp,
+                                   ctypes.POINTER(MatchingInfo), _vp, ctypes.c_longlong, _vp]
+lib.dt_matching_device.restype = ctypes.c_int
+lib.dt_matching_host.argtypes = [_sz, _sz, _sz, _f, _f, _f, _vp, _f, _f, _f, ctypes.c_longlong, _vp,
+                                 ctypes.POINTER(MatchingInfo), _vp, ctypes.c_longlong]
+lib.dt_matching_host.restype = ctypes.c_int
 
 
 class EngineConfig(ctypes.Structure):
@@ -269,6 +297,39 @@ def solve_device(algorithm, nx, ny, nz, spacing, d_target, d_mask, d_field, stre
     rc = lib.dt_solve_device(ALGORITHMS[algorithm], nx, ny, nz, *map(float, spacing),
                              d_target, d_mask, ctypes.byref(p), d_field, ctypes.byref(info),
                              _host_ptr(trace) if trace_cap else None, trace_cap, stream or None)
+    if rc != 0:
+        raise HipError(last_error())
+    return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def matching_host(target, spacing=(1.0, 1.0, 1.0), field=None, it_max=50000, trace_cap=0, epsilon=9.99e-4,
+                  delta=1e-3, zeta=10.0):
+    """The matching algorithm with host numpy buffers; returns (field, info,
+    trace rows (it, cost, max voxel error, step))."""
+    t = np.ascontiguousarray(target, np.float32)
+    nz, ny, nx = t.shape
+    f = (np.zeros((3, nz, ny, nx), np.float32) if field is None
+         else np.array(field, np.float32, order="C", copy=True))
+    info = MatchingInfo()
+    trace = np.zeros((max(trace_cap, 1), 4), np.float64)
+    rc = lib.dt_matching_host(nx, ny, nz, *map(float, spacing), _host_ptr(t), float(epsilon), float(delta),
+                              float(zeta), int(it_max), _host_ptr(f), ctypes.byref(info),
+                              _host_ptr(trace) if trace_cap else None, trace_cap)
+    if rc != 0:
+        raise HipError(last_error())
+    return f, info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def matching_device(nx, ny, nz, spacing, d_target, d_field, stream=0, it_max=50000, trace_cap=0,
+                    epsilon=9.99e-4, delta=1e-3, zeta=10.0, **unused):
+    """The matching algorithm on device pointers (ints); returns (info, trace).
+    Solver parameters other than epsilon, delta, zeta and it_max are accepted
+    and unused, as in the reference."""
+    info = MatchingInfo()
+    trace = np.zeros((max(trace_cap, 1), 4), np.float64)
+    rc = lib.dt_matching_device(nx, ny, nz, *map(float, spacing), d_target, float(epsilon), float(delta),
+                                float(zeta), int(it_max), d_field, ctypes.byref(info),
+                                _host_ptr(trace) if trace_cap else None, trace_cap, stream or None)
     if rc != 0:
         raise HipError(last_error())
     return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()

@@ -125,6 +125,48 @@ typedef struct dt_solve_info {
                                 0 in engine results)                        */
 } dt_solve_info;
 
+/* Record of a matching solve (dt_matching_device / dt_matching_host). */
+typedef struct dt_matching_info {
+    long long iterations;       /* accepted iterations                         */
+    int reason;                 /* DT_STOP_MAX_ITERATIONS, DT_STOP_ETA_SMALL
+                                   (step below 1e-9) or DT_STOP_TOLERANCE
+                                   (max voxel error <= epsilon)                */
+    long long cost_passes;      /* cost evaluations (trials + the initial one) */
+    long long gradient_passes;  /* gradient evaluations                        */
+    long long launches;         /* pass kernels launched (incl. idle ones)     */
+    double final_cost;          /* cost of the returned field (FP64 sum)       */
+    double final_max_error;     /* max voxel error of the last evaluated trial */
+    long long trace_len;        /* log lines (it, cost, max error, step)       */
+    double seconds;             /* wall time of the solve                      */
+    double kernel_seconds;      /* device time of the pass launches            */
+} dt_matching_info;
+
+/* The `matching` algorithm (src/VolumeMatching3D.c) on device buffers:
+ * d_target [nz][ny][nx], d_field [3][nz][ny][nx] in/out.  Uses epsilon,
+ * delta, zeta and it_max (the reference ignores the other solver parameters
+ * and the mask).  `trace` (host, optional) receives up to trace_cap rows of 4
+ * doubles.  Runs on `stream` and synchronises it.  Returns 0 or -1. */
+int dt_matching_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                       const float *d_target, float epsilon, float delta, float zeta,
+                       long long it_max, float *d_field, dt_matching_info *info,
+                       double *trace, long long trace_cap, void *stream);
+
+/* Same with host buffers (one H2D and one D2H copy). */
+int dt_matching_host(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                     const float *target, float epsilon, float delta, float zeta,
+                     long long it_max, float *field, dt_matching_info *info,
+                     double *trace, long long trace_cap);
+
+/* Reference entry point of the matching algorithm
+ * (src/headers/VolumeMatching3D.h:7-29, src/VolumeMatching3D.c:689-796), run on
+ * the current HIP device; host buffers, field in place, errors in the error
+ * state.  CPU-only in the reference. */
+void volume_matching_3d(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                        const float *J, const bool *mask, float epsilon, float tolerance,
+                        float eta, float eta_max, float alpha, float beta, float gamma,
+                        float delta, float zeta, float theta, float iota, bool strict,
+                        size_t it_max, float *field);
+
 /* Jacobian of a device-resident field (device pointers). Returns 0 or -1. */
 int dt_jacobian_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
                        const float *d_field, float *d_J, void *stream);

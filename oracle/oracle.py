@@ -8,7 +8,8 @@ Two libraries are bound:
 * ``Oracle`` -- this repository's C99 restatement (``oracle/dt_oracle.c``),
   built by ``oracle/Makefile`` into ``oracle/build/liboracle_f{32,64}.so``;
 * ``Reference`` -- the reference C library compiled from its own sources
-  (``src/{disptools,jacobian,displacement_field_gradient,displacement_field_greedy}.c``)
+  (``src/{disptools,jacobian,displacement_field_gradient,displacement_field_greedy,
+  VolumeMatching3D}.c``)
   into ``oracle/_ref/libdtref_f{32,64}.so``.  It prints one log line per
   iteration to the C ``stdout`` (``displacement_field_gradient.c:272-277``,
   ``displacement_field_greedy.c:184-189``); ``Reference.solve`` captures and
@@ -59,6 +60,20 @@ class _Result(ctypes.Structure):
         ("evaluations", ctypes.c_long),
         ("rejections", ctypes.c_long),
         ("final_error", ctypes.c_double),
+        ("final_max_error", ctypes.c_double),
+        ("trace", ctypes.POINTER(ctypes.c_double)),
+        ("trace_cap", ctypes.c_long),
+        ("trace_len", ctypes.c_long),
+    ]
+
+
+class _MatchResult(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_long),
+        ("reason", ctypes.c_int),
+        ("cost_evaluations", ctypes.c_long),
+        ("gradient_evaluations", ctypes.c_long),
+        ("final_cost", ctypes.c_double),
         ("final_max_error", ctypes.c_double),
         ("trace", ctypes.POINTER(ctypes.c_double)),
         ("trace_cap", ctypes.c_long),
@@ -127,6 +142,13 @@ class Oracle(_Base):
             getattr(L, name).restype = ctypes.c_int
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = None
+        L.oracle_match_cost.argtypes = [sz, sz, sz, vp, vp, vp, vp, R, R, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_match_cost.restype = ctypes.c_double
+        L.oracle_match_grad.argtypes = [sz, sz, sz, vp, vp, vp, vp, R, R, vp, vp, vp]
+        L.oracle_match_grad.restype = None
+        L.oracle_solve_matching.argtypes = [sz, sz, sz, R, R, R, vp, R, R, R, sz, vp,
+                                            ctypes.POINTER(_MatchResult)]
+        L.oracle_solve_matching.restype = ctypes.c_int
 
     def set_threads(self, n):
         self.lib.oracle_set_threads(int(n))
@@ -190,6 +212,48 @@ class Oracle(_Base):
         return f, info
 
 
+    # -- matching (src/VolumeMatching3D.c) -------------------------------------
+
+    def match_cost(self, F, G, E, V, delta=1e-3, zeta=10.0):
+        """(cost, max voxel error) of node positions F, G, E [nz, ny, nx] and
+        cell targets V [nz, ny, nx] (the target at each cell's lowest node)."""
+        a = [np.ascontiguousarray(x, self.dtype) for x in (F, G, E, V)]
+        nz, ny, nx = a[0].shape
+        mx = ctypes.c_double(0)
+        c = self.lib.oracle_match_cost(nx, ny, nz, *map(_ptr, a), delta, zeta, ctypes.byref(mx))
+        return float(c), float(mx.value)
+
+    def match_grad(self, F, G, E, V, delta=1e-3, zeta=10.0):
+        """Gradient (dF, dG, dE) of match_cost."""
+        a = [np.ascontiguousarray(x, self.dtype) for x in (F, G, E, V)]
+        nz, ny, nx = a[0].shape
+        d = [np.zeros((nz, ny, nx), self.dtype) for _ in range(3)]
+        self.lib.oracle_match_grad(nx, ny, nz, *map(_ptr, a), delta, zeta, *map(_ptr, d))
+        return d
+
+    def matching(self, target, spacing=(1.0, 1.0, 1.0), field=None, it_max=100, trace_cap=100000,
+                 epsilon=9.99e-4, delta=1e-3, zeta=10.0):
+        """The matching solve; returns (field, info) with the trace rows
+        (it, cost, max voxel error, step) of the accepted iterations."""
+        t = np.ascontiguousarray(target, self.dtype)
+        nz, ny, nx = t.shape
+        f = (np.zeros((3, nz, ny, nx), self.dtype) if field is None
+             else self._field(field).copy())
+        trace = np.zeros((max(trace_cap, 1), 4), np.float64)
+        res = _MatchResult()
+        res.trace = trace.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        res.trace_cap = trace_cap
+        rc = self.lib.oracle_solve_matching(nx, ny, nz, *spacing, _ptr(t), epsilon, delta, zeta, it_max,
+                                            _ptr(f), ctypes.byref(res))
+        if rc != 0:
+            raise MemoryError("oracle matching could not allocate scratch memory")
+        info = dict(iterations=res.iterations, reason=res.reason, cost_evaluations=res.cost_evaluations,
+                    gradient_evaluations=res.gradient_evaluations, final_cost=res.final_cost,
+                    final_max_error=res.final_max_error,
+                    trace=trace[:min(res.trace_len, trace_cap)].copy())
+        return f, info
+
+
 _LINE = re.compile(r"Iteration\s+(-?\d+):\s+total error\s+(\S+)\s+max voxel error\s+(\S+)\s+eta\s+(\S+)")
 
 
@@ -207,7 +271,38 @@ class Reference(_Base):
         for name in ("generate_displacement_gradient", "generate_displacement_greedy"):
             getattr(L, name).argtypes = solver
             getattr(L, name).restype = None
+        self.has_matching = hasattr(L, "volume_matching_3d")
+        if self.has_matching:
+            L.volume_matching_3d.argtypes = solver
+            L.volume_matching_3d.restype = None
         self._libc = ctypes.CDLL(None)
+
+    def matching(self, target, mask=None, spacing=(1.0, 1.0, 1.0), field=None, it_max=100, **params):
+        """volume_matching_3d (src/VolumeMatching3D.c:689-796); the log lines
+        of the accepted iterations are parsed into info['trace']."""
+        t = np.ascontiguousarray(target, self.dtype)
+        nz, ny, nx = t.shape
+        m = (np.ones(t.shape, np.bool_) if mask is None else np.ascontiguousarray(mask, np.bool_))
+        f = (np.zeros((3, nz, ny, nx), self.dtype) if field is None
+             else self._field(field).copy())
+        args, strict = self._solver_args(params)
+        call = lambda: self.lib.volume_matching_3d(nx, ny, nz, *spacing, _ptr(t), _ptr(m), *args, strict,
+                                                   it_max, _ptr(f))
+        # The reference takes its gradient buffers from malloc without clearing
+        # them before the first gradient (VolumeMatching3D.c:422-424; they are
+        # zeroed only after an accepted step, :532-552), so when malloc recycles
+        # freed heap memory the first step picks up stale values.  While it
+        # runs, glibc's perturb option (M_PERTURB, mallopt) fills every block
+        # malloc hands out with the byte 255 ^ 0xff = 0: the reference then
+        # computes what it computes on fresh memory, a zero-initialised gradient.
+        M_PERTURB = -6
+        self._libc.mallopt(M_PERTURB, 255)
+        try:
+            text = self._with_stdout(call, True)
+        finally:
+            self._libc.mallopt(M_PERTURB, 0)
+        rows = [[float(mm.group(i)) for i in range(1, 5)] for mm in map(_LINE.search, text.splitlines()) if mm]
+        return f, dict(trace=np.array(rows, np.float64).reshape(-1, 4), text=text)
 
     def jacobian(self, field, spacing=(1.0, 1.0, 1.0)):
         f = self._field(field)
