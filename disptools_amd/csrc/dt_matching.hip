@@ -43,12 +43,10 @@
 namespace dt {
 namespace {
 
-constexpr int MX = 32, MY = 8, MNT = MX * MY;    // column tile
-constexpr int CPX = MX + 1, CPP = CPX * (MY + 1);  // cost pass: node plane with +1 halo
-constexpr int GPX = MX + 2, GPP = GPX * (MY + 2);  // gradient pass: node plane with a 1-ring
-constexpr int NGRING = GPP - MNT;                  // 2 MX + 2 MY + 4
-constexpr int NCRING = CPP - MNT;                  // MX + MY + 1
-static_assert(NGRING <= MNT && NCRING <= MNT, "one extra point per thread at most");
+constexpr int MX = 32, MY = 16, MNT = MX * MY;    // cell tile, one thread per cell
+constexpr int NPX = MX + 1, NPP = NPX * (MY + 1);  // nodes of a cell tile's plane
+constexpr int NEXTRA = NPP - MNT;                  // node points loaded by a second slot
+static_assert(NEXTRA <= MNT, "one extra point per thread at most");
 
 enum { MPH_INIT = 0, MPH_GRAD = 1, MPH_COST = 2, MPH_DONE = 3 };
 enum { MM_FIRST = 0, MM_GROW = 1, MM_SHRINK = 2 };
@@ -62,8 +60,13 @@ struct MCtrl {
     int reason;
 };
 
+struct PassGeom {
+    int ntx, nty, ntz, tz;
+};
+
 struct MInst {
-    int nx, ny, nz, tz, ntx, nty, ntz;
+    int nx, ny, nz;
+    PassGeom cg, gg;           // tilings of the cost and gradient passes
     float delta, zeta, epsilon;
     long long it_max;
     long long n;
@@ -71,7 +74,7 @@ struct MInst {
     float *w[3];               // cell weights of triple t, node-indexed
     float *D;                  // gradient, F, G, E at D + {0, n, 2n}
     const float *V;            // target, node-indexed
-    double *partials;          // 2 per tile
+    double *partials;          // 2 per block
     unsigned *counter;
     MCtrl *ctrl;
     double *trace;
@@ -126,17 +129,31 @@ __device__ __forceinline__ long long node_index(int nx, int ny, int i, int j, in
 // ---------------------------------------------------------------------------
 
 struct CostSmem {
-    float s[3][3][CPP];        // trial node planes, ring of three
+    float s[3][3][NPP];        // trial node planes, ring of three
 };
 struct GradSmem {
-    float x[4][3][GPP];        // accepted node planes, ring of four
-    float w[3][CPP];           // cell weights, ring of three
+    float x[2][3][NPP];        // accepted node planes, ring of two
+    float c[8][3][MNT];        // each cell's contributions to its 8 nodes
 };
 union PassSmem {
     CostSmem c;
     GradSmem g;
 };
 
+// Node points of a thread: its own (lx, ly) and, for the first NEXTRA
+// threads, one of the row MY / column MX that completes the (MX+1) x (MY+1)
+// node plane of the cell tile.
+__device__ __forceinline__ void node_points(int tid, int (&px)[2], int (&py)[2], bool &has1)
+{
+    px[0] = tid % MX;
+    py[0] = tid / MX;
+    has1 = tid < NEXTRA;
+    if (tid < MX + 1) { px[1] = tid; py[1] = MY; }
+    else { px[1] = MX; py[1] = tid - (MX + 1); }
+}
+
+// Cost pass over the cells of a tile whose lowest cell is (x0, y0); stores the
+// trial at the tile's nodes [x0, x0+MX) x [y0, y0+MY) and planes [z0, z1).
 __device__ void cost_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0, int z1, PassSmem &S,
                           double &Hs, float &mx)
 {
@@ -150,12 +167,9 @@ __device__ void cost_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0
     const float alpha = C.alpha;
     const int tid = threadIdx.x;
     const int lx = tid % MX, ly = tid / MX;
-    // node points: own, and the +1 halo row / column
     int px[2], py[2];
-    px[0] = lx; py[0] = ly;
-    const bool has1 = tid < NCRING;
-    if (tid < MX + 1) { px[1] = tid; py[1] = MY; }
-    else { px[1] = MX; py[1] = tid - (MX + 1); }
+    bool has1;
+    node_points(tid, px, py, has1);
     bool ok[2];
     long long nid[2];
     int li[2];
@@ -164,40 +178,51 @@ __device__ void cost_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0
         const int gx = x0 + px[k], gy = y0 + py[k];
         ok[k] = (k == 0 || has1) && gx < nx && gy < ny;
         nid[k] = ok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
-        li[k] = py[k] * CPX + px[k];
+        li[k] = py[k] * NPX + px[k];
     }
     const int gx = x0 + lx, gy = y0 + ly;
     const bool cell_ok = gx < nx - 1 && gy < ny - 1;
     const long long plane = (long long)nx * ny;
     const int pend = min(z1, nz - 1);
+    // raw loads of the next node plane
+    float xr[2][3], dr[2][3];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const long long id = c * n + nid[k] + p * plane;
+                xr[k][c] = ok[k] ? X[id] : 0.0f;
+                dr[k][c] = (ok[k] && !base) ? D[id] : 0.0f;
+            }
+    };
+    fetch(z0);
     for (int p = z0; p <= pend; ++p) {
-        float(*dst)[CPP] = S.c.s[p % 3];
+        float(*dst)[NPP] = S.c.s[p % 3];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (!(k == 0 || has1)) continue;
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            if (ok[k]) {
-                const long long id = nid[k] + p * plane;
+            float v[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    v[c] = base ? X[c * n + id] : step_value(X[c * n + id], alpha, D[c * n + id]);
-                if (k == 0 && T && p < z1) {
+            for (int c = 0; c < 3; ++c) v[c] = base ? xr[k][c] : step_value(xr[k][c], alpha, dr[k][c]);
+            if (k == 0 && T && ok[0] && p < z1) {
+                const long long id = nid[0] + p * plane;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) T[c * n + id] = v[c];
-                }
+                for (int c = 0; c < 3; ++c) T[c * n + id] = v[c];
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) dst[c][li[k]] = v[c];
         }
+        if (p < pend) fetch(p + 1);
         __syncthreads();
         const int kc = p - 1;
         if (kc >= z0 && cell_ok) {
-            const float(*lo)[CPP] = S.c.s[kc % 3];
-            const float(*hi)[CPP] = S.c.s[p % 3];
+            const float(*lo)[NPP] = S.c.s[kc % 3];
+            const float(*hi)[NPP] = S.c.s[p % 3];
             float v[3][8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int idx = (ly + ((q >> 1) & 1)) * CPX + lx + (q & 1);
+                const int idx = (ly + ((q >> 1) & 1)) * NPX + lx + (q & 1);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c][q] = (q & 4) ? hi[c][idx] : lo[c][idx];
             }
@@ -227,7 +252,14 @@ __device__ void cost_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0
     }
 }
 
-__device__ void grad_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0, int z1, PassSmem &S)
+// Gradient pass.  The tile's cells start at (cx0, cy0) = (tile * (MX-1) - 1,
+// ...), one thread per cell; a thread with lx, ly >= 1 owns node (cx0+lx,
+// cy0+ly), whose in-plane cells are the threads (lx-1..lx, ly-1..ly).  Per
+// cell plane p the cell's corners give their weighted cofactors to the cell's
+// 8 nodes (each corner's matrix and cofactors computed once); the node on
+// plane p sums what the cells of planes p-1 (kept in a register) and p give
+// it, in cell order.
+__device__ void grad_pass(const MInst &I, const MCtrl &C, int cx0, int cy0, int z0, int z1, PassSmem &S)
 {
     const int nx = I.nx, ny = I.ny, nz = I.nz;
     const long long n = I.n;
@@ -238,113 +270,122 @@ __device__ void grad_pass(const MInst &I, const MCtrl &C, int x0, int y0, int z0
     const int tid = threadIdx.x;
     const int lx = tid % MX, ly = tid / MX;
     const long long plane = (long long)nx * ny;
-    // node points in ring coordinates [0, MX+2) x [0, MY+2): own (lx+1, ly+1),
-    // and the ring around the tile
     int px[2], py[2];
-    px[0] = lx + 1; py[0] = ly + 1;
-    const bool has1 = tid < NGRING;
-    {
-        const int r = tid;
-        if (r < GPX) { px[1] = r; py[1] = 0; }
-        else if (r < 2 * GPX) { px[1] = r - GPX; py[1] = MY + 1; }
-        else { const int q = r - 2 * GPX; px[1] = (q & 1) ? MX + 1 : 0; py[1] = 1 + (q >> 1); }
-    }
+    bool has1;
+    node_points(tid, px, py, has1);
     bool ok[2];
     long long nid[2];
     int li[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        const int gx = x0 - 1 + px[k], gy = y0 - 1 + py[k];
+        const int gx = cx0 + px[k], gy = cy0 + py[k];
         ok[k] = (k == 0 || has1) && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         nid[k] = ok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
-        li[k] = py[k] * GPX + px[k];
+        li[k] = py[k] * NPX + px[k];
     }
-    // cell points in cell coordinates [0, MX+1) x [0, MY+1) (cell x0-1+cx):
-    // own (lx+1, ly+1), and the low row / column
-    int cx[2], cy[2];
-    cx[0] = lx + 1; cy[0] = ly + 1;
-    const bool hasc = tid < NCRING;
-    if (tid < MX + 1) { cx[1] = tid; cy[1] = 0; }
-    else { cx[1] = 0; cy[1] = tid - MX; }
-    bool cok[2];
-    long long cid[2];
-    int ci[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int gx = x0 - 1 + cx[k], gy = y0 - 1 + cy[k];
-        cok[k] = (k == 0 || hasc) && gx >= 0 && gx < nx - 1 && gy >= 0 && gy < ny - 1;
-        cid[k] = cok[k] ? node_index(nx, ny, gx, gy, 0) : 0;
-        ci[k] = cy[k] * CPX + cx[k];
-    }
-    const int gx = x0 + lx, gy = y0 + ly;
-    const bool own = gx < nx && gy < ny;
+    const int ci = cx0 + lx, cj = cy0 + ly;
+    const bool cell_in = ci >= 0 && ci < nx - 1 && cj >= 0 && cj < ny - 1;
+    const long long cid = cell_in ? node_index(nx, ny, ci, cj, 0) : 0;
+    const bool owner = lx >= 1 && ly >= 1 && ci < nx && cj < ny;
 
-    for (int p = z0 - 1; p <= z1; ++p) {
-        // node plane p and cell plane p - 1 join the rings
+    float xr[2][3];
+    auto fetch_x = [&](int p) {
         const bool zin = p >= 0 && p < nz;
-        float(*xd)[GPP] = S.g.x[(p + 4) % 4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                xr[k][c] = (zin && ok[k]) ? X[c * n + nid[k] + (long long)min(max(p, 0), nz - 1) * plane] : 0.0f;
+    };
+    auto fetch_w = [&](int p) -> float {
+        return (cell_in && p >= 0 && p < nz - 1) ? Wc[cid + (long long)min(max(p, 0), nz - 1) * plane] : 0.0f;
+    };
+    auto put_x = [&](int p) {
+        float(*dst)[NPP] = S.g.x[(p + 2) & 1];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (!(k == 0 || has1)) continue;
-            const long long id = nid[k] + (long long)min(max(p, 0), nz - 1) * plane;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) xd[c][li[k]] = (zin && ok[k]) ? X[c * n + id] : 0.0f;
+            for (int c = 0; c < 3; ++c) dst[c][li[k]] = xr[k][c];
         }
-        const int pc = p - 1;
-        const bool cin = pc >= 0 && pc < nz - 1;
-        float *wd = S.g.w[(pc + 3) % 3];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (!(k == 0 || hasc)) continue;
-            wd[ci[k]] = (cin && cok[k]) ? Wc[cid[k] + (long long)min(max(pc, 0), nz - 1) * plane] : 0.0f;
+    };
+    // node plane z0-1 into LDS; node plane z0 and the weights of cell plane
+    // z0-1 in flight
+    fetch_x(z0 - 1);
+    put_x(z0 - 1);
+    fetch_x(z0);
+    float wnext = fetch_w(z0 - 1);
+    float up[3] = {0.0f, 0.0f, 0.0f};
+    for (int p = z0 - 1; p < z1; ++p) {
+        put_x(p + 1);
+        const float wcell = wnext;
+        if (p + 1 < z1) {
+            fetch_x(p + 2);
+            wnext = fetch_w(p + 1);
         }
         __syncthreads();
-        const int kk = p - 1;   // node plane evaluated at this step
-        if (kk < z0 || kk >= z1 || !own) continue;
-        float acc[3] = {0.0f, 0.0f, 0.0f};
-        // cells (i-1+ax, j-1+ay, k-1+az), in cell order
+        // contributions of cell (ci, cj, p) to its 8 nodes
+        float acc[8][3];
 #pragma unroll
-        for (int az = 0; az < 2; ++az)
+        for (int q = 0; q < 8; ++q) acc[q][0] = acc[q][1] = acc[q][2] = 0.0f;
+        if (cell_in && p >= 0 && p < nz - 1) {
+            float v[3][8];
 #pragma unroll
-            for (int ay = 0; ay < 2; ++ay)
+            for (int q = 0; q < 8; ++q) {
+                const float(*xs)[NPP] = S.g.x[(p + 2 + ((q >> 2) & 1)) & 1];
+                const int idx = (ly + ((q >> 1) & 1)) * NPX + lx + (q & 1);
 #pragma unroll
-                for (int ax = 0; ax < 2; ++ax) {
-                    const int ck = kk - 1 + az, cj = gy - 1 + ay, cc = gx - 1 + ax;
-                    if (ck < 0 || ck >= nz - 1 || cj < 0 || cj >= ny - 1 || cc < 0 || cc >= nx - 1) continue;
-                    const float wcell = S.g.w[(ck + 3) % 3][(ly + ay) * CPX + lx + ax];
-                    float v[3][8];
+                for (int c = 0; c < 3; ++c) v[c][q] = xs[c][idx];
+            }
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        const float(*xs)[GPP] = S.g.x[(ck + ((q >> 2) & 1) + 4) % 4];
-                        const int idx = (ly + ay + ((q >> 1) & 1)) * GPX + lx + ax + (q & 1);
+            for (int q = 0; q < 8; ++q) {
+                float m[3][3], cf[3][3];
+                corner_matrix(v, q, m);
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) v[c][q] = xs[c][idx];
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) cf[r][c] = cof(m, r, c);
+                const float J = m[0][0] * cf[0][0] + m[0][1] * cf[0][1] + m[0][2] * cf[0][2];
+                const float Wq = wcell + (J < delta ? zeta * (J - delta) : 0.0f);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int b = COL_BIT[c];
+                    const float sgn = (q & b) ? 1.0f : -1.0f;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const float t = sgn * (Wq * cf[r][c]);
+                        acc[q][r] += t;
+                        acc[q ^ b][r] -= t;
                     }
-                    // the node is corner p of this cell
-                    const int pq = (1 - ax) | ((1 - ay) << 1) | ((1 - az) << 2);
-                    float m0[3][3];
-                    corner_matrix(v, pq, m0);
-                    const float J0 = det3(m0);
-                    const float W0 = wcell + (J0 < delta ? zeta * (J0 - delta) : 0.0f);
-                    float part[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-                    for (int col = 0; col < 3; ++col) {
-                        const int b = COL_BIT[col];
-                        const int qn = pq ^ b;
-                        float mn[3][3];
-                        corner_matrix(v, qn, mn);
-                        const float Jn = det3(mn);
-                        const float Wn = wcell + (Jn < delta ? zeta * (Jn - delta) : 0.0f);
-                        const float sgn = (pq & b) ? 1.0f : -1.0f;
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) part[r] += sgn * (W0 * cof(m0, r, col) + Wn * cof(mn, r, col));
-                    }
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) acc[r] += part[r];
                 }
-        const long long id = node_index(nx, ny, gx, gy, kk);
+            }
+        }
 #pragma unroll
-        for (int r = 0; r < 3; ++r) D[r * n + id] = acc[r];
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) S.g.c[q][r][tid] = acc[q][r];
+        __syncthreads();
+        if (owner) {
+            // cells (lx-1, ly-1), (lx, ly-1), (lx-1, ly), (lx, ly): the node is
+            // their corner 3, 2, 1, 0 (+4 for the plane above)
+            const int cell[4] = {tid - MX - 1, tid - MX, tid - 1, tid};
+            if (p >= z0) {
+                float d[3] = {up[0], up[1], up[2]};
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) d[r] += S.g.c[3 - t][r][cell[t]];
+                const long long id = node_index(nx, ny, ci, cj, p);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) D[r * n + id] = d[r];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) up[r] = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) up[r] += S.g.c[7 - t][r][cell[t]];
+        }
     }
 }
 
@@ -460,13 +501,17 @@ k_match_pass(const MInst *__restrict__ pinst)
     MCtrl *C = I.ctrl;
     if (!C->active) return;
     const MCtrl ctl = *C;
+    const bool grad = ctl.phase == MPH_GRAD;
+    const PassGeom G = grad ? I.gg : I.cg;
     const int t = blockIdx.x;
-    const int tx = t % I.ntx, ty = (t / I.ntx) % I.nty, tz = t / (I.ntx * I.nty);
-    const int x0 = tx * MX, y0 = ty * MY, z0 = tz * I.tz, z1 = min(z0 + I.tz, I.nz);
     double Hs = 0.0;
     float mx = 0.0f;
-    if (ctl.phase == MPH_GRAD) grad_pass(I, ctl, x0, y0, z0, z1, S);
-    else cost_pass(I, ctl, x0, y0, z0, z1, S, Hs, mx);
+    if (t < G.ntx * G.nty * G.ntz) {
+        const int tx = t % G.ntx, ty = (t / G.ntx) % G.nty, tz = t / (G.ntx * G.nty);
+        const int z0 = tz * G.tz, z1 = min(z0 + G.tz, I.nz);
+        if (grad) grad_pass(I, ctl, tx * (MX - 1) - 1, ty * (MY - 1) - 1, z0, z1, S);
+        else cost_pass(I, ctl, tx * MX, ty * MY, z0, z1, S, Hs, mx);
+    }
 
     // block partials, then the last block folds all tiles in order
 #pragma unroll
@@ -543,19 +588,30 @@ unsigned grid_of(long long work)
     return (unsigned)std::max<long long>(1, std::min<long long>(b, 65536));
 }
 
-// z-chunk depth: about 8 blocks per CU in one round
-int match_tz(int nx, int ny, int nz)
+// Tiling of a pass whose tiles cover sx x sy nodes in the plane: z-chunks as
+// deep as keep all tiles within one round of resident blocks.
+PassGeom pass_geom(int nx, int ny, int nz, int sx, int sy, int resident)
 {
-    int dev = 0, cus = 256;
+    PassGeom g;
+    g.ntx = (nx + sx - 1) / sx;
+    g.nty = (ny + sy - 1) / sy;
+    const long long cols = (long long)g.ntx * g.nty;
+    const long long chunks = std::max<long long>(1, std::min<long long>(nz, resident / cols));
+    g.tz = (int)((nz + chunks - 1) / chunks);
+    g.ntz = (nz + g.tz - 1) / g.tz;
+    return g;
+}
+
+int resident_match_blocks()
+{
+    int dev = 0, cus = 256, per = 0;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
-    const long long cols = (long long)((nx + MX - 1) / MX) * ((ny + MY - 1) / MY);
-    const long long want = 8ll * cus;
-    const long long chunks = std::max<long long>(1, std::min<long long>(nz, (want + cols - 1) / cols));
-    return std::max(2, (int)((nz + chunks - 1) / chunks));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_match_pass, MNT, 0) != hipSuccess || per < 1) per = 1;
+    return cus * per;
 }
 
 struct MatchHost {
@@ -574,14 +630,13 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     const long long n = (long long)nx * ny * nz;
     MInst h{};
     h.nx = nx; h.ny = ny; h.nz = nz;
-    h.tz = match_tz(nx, ny, nz);
-    h.ntx = (nx + MX - 1) / MX;
-    h.nty = (ny + MY - 1) / MY;
-    h.ntz = (nz + h.tz - 1) / h.tz;
+    const int resident = resident_match_blocks();
+    h.cg = pass_geom(nx, ny, nz, MX, MY, resident);
+    h.gg = pass_geom(nx, ny, nz, MX - 1, MY - 1, resident);
     h.delta = delta; h.zeta = zeta; h.epsilon = epsilon;
     h.it_max = it_max < 0 ? 0 : it_max;
     h.n = n;
-    const int ntiles = h.ntx * h.nty * h.ntz;
+    const int ntiles = std::max(h.cg.ntx * h.cg.nty * h.cg.ntz, h.gg.ntx * h.gg.nty * h.gg.ntz);
     if (trace_cap < 0 || !trace) trace_cap = 0;
 
     size_t bytes = 0;
