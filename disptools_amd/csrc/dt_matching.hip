@@ -503,7 +503,11 @@ k_match_pass(const MInst *__restrict__ pinst)
     const MCtrl ctl = *C;
     const bool grad = ctl.phase == MPH_GRAD;
     const PassGeom G = grad ? I.gg : I.cg;
-    const int t = blockIdx.x;
+    // blocks are dispatched to the 8 XCDs round robin: give each XCD a
+    // contiguous range of tiles, so that neighbouring tiles share an L2 and
+    // their halo loads hit it (the grid is a multiple of 8)
+    const int per_xcd = (int)gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     double Hs = 0.0;
     float mx = 0.0f;
     if (t < G.ntx * G.nty * G.ntz) {
@@ -636,7 +640,7 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     h.delta = delta; h.zeta = zeta; h.epsilon = epsilon;
     h.it_max = it_max < 0 ? 0 : it_max;
     h.n = n;
-    const int ntiles = std::max(h.cg.ntx * h.cg.nty * h.cg.ntz, h.gg.ntx * h.gg.nty * h.gg.ntz);
+    const int ntiles = (std::max(h.cg.ntx * h.cg.nty * h.cg.ntz, h.gg.ntx * h.gg.nty * h.gg.ntz) + 7) / 8 * 8;
     if (trace_cap < 0 || !trace) trace_cap = 0;
 
     size_t bytes = 0;
