@@ -108,6 +108,10 @@ __device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, int 
     return K;
 }
 
+// Fused multiply-add in the arithmetic type (__builtin_fma is the double one).
+__device__ __forceinline__ float fma_ct(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_ct(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // J - 1 from the column's coefficients (planes 2 .. nz - 3).  A lane with the
 // 4th-order stencil rounds as jacobian_delta's interior form (G c = 0 adds
 // an exact zero).
@@ -122,7 +126,7 @@ __device__ __forceinline__ float jacobian_delta_coef(const ColCoef &K, const flo
         for (int a = 0; a < 3; ++a) {
             const CT D1 = (CT)(sm[c][a][2] - sm[c][a][1]);
             const CT D2 = (CT)(sm[c][a][3] - sm[c][a][0]);
-            d4[c][a] = __builtin_fma(D1, (CT)K.A[a], __builtin_fma(D2, (CT)K.B[a], (CT)K.G[a] * (CT)v0[c]));
+            d4[c][a] = fma_ct(D1, (CT)K.A[a], fma_ct(D2, (CT)K.B[a], (CT)K.G[a] * (CT)v0[c]));
         }
     return (float)det_plus_identity_minus_one<CT>(d4);
 }
