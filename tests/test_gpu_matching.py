@@ -140,3 +140,20 @@ def test_matching_device_buffers(gpu_lib):
     assert info["iterations"] == ih["iterations"]
     assert np.array_equal(dF.cpu().numpy(), fh)
     assert info["kernel_seconds"] > 0
+
+
+def test_python_displacement_matching(gpu_lib, oracle64):
+    """displacement(..., algorithm='matching') pads, solves on the GPU and
+    crops, like the other algorithms (reference displacements.py:443-483)."""
+    from disptools_amd import displacements
+    from disptools_amd.workloads import sphere_case
+    T, M, sp = sphere_case(20, radius=5)
+    out = displacements.displacement(T, mask=M, it_max=40, algorithm="matching", gpu_id=0)
+    assert out.shape == (20, 20, 20, 3)
+    Tp = np.pad(T, 1, mode="edge")
+    fo, io = oracle64.matching(Tp.astype(np.float64), it_max=40)
+    fo = np.moveaxis(fo, 0, -1)[1:-1, 1:-1, 1:-1]
+    assert np.abs(out - fo).max() < 5e-4
+    # two levels: the pyramid path runs matching on device buffers
+    out2 = displacements.displacement(T, mask=M, it_max=[20, 20], levels=2, algorithm="matching", gpu_id=0)
+    assert out2.shape == (20, 20, 20, 3) and np.isfinite(out2).all()
