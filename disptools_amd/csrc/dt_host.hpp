@@ -70,10 +70,11 @@ struct Plan {
     long long n = 0;
 };
 
-// Blocks of one pass launch resident at once on the current device.
-int resident_blocks(int precision, int algorithm, int variant);
-// z-chunk depth that minimises (rounds of resident blocks) x (TZ + 3).
-int choose_tz(int nx, int ny, int nz, int tx, int ty, int resident);
+// Compute units of the current device.
+int device_cus();
+// z-chunk depth that minimises (blocks per CU, or rounds of resident blocks)
+// x (TZ + overhead).
+int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int overhead, bool shared_cu);
 // DISPTOOLS_GRAD_VARIANT / DISPTOOLS_GREEDY_VARIANT / DISPTOOLS_STENCIL_PRECISION overrides.
 int variant_from_env();
 int greedy_variant_from_env();
