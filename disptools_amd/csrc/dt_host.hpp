@@ -70,6 +70,10 @@ struct Plan {
     long long n = 0;
 };
 
+// Per-iteration log lines of the reference entry points (DISPTOOLS_VERBOSE).
+bool verbose_from_env();
+void print_iteration(long long it, double E, double m, double eta);
+
 // Compute units of the current device.
 int device_cus();
 // z-chunk depth that minimises (blocks per CU, or rounds of resident blocks)

@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -844,6 +845,20 @@ extern "C" void volume_matching_3d(size_t nx, size_t ny, size_t nz, float dx, fl
 {
     (void)mask; (void)tolerance; (void)eta; (void)eta_max; (void)alpha; (void)beta; (void)gamma;
     (void)theta; (void)iota; (void)strict;
-    (void)dt_matching_host(nx, ny, nz, dx, dy, dz, J, epsilon, delta, zeta, (long long)it_max, field, nullptr,
-                           nullptr, 0);
+    if (!verbose_from_env()) {
+        (void)dt_matching_host(nx, ny, nz, dx, dy, dz, J, epsilon, delta, zeta, (long long)it_max, field, nullptr,
+                               nullptr, 0);
+        return;
+    }
+    // the reference prints one line per accepted iteration (VolumeMatching3D.c:538-544)
+    const long long cap = std::min<long long>((long long)it_max + 1, 1ll << 21);
+    std::vector<double> tr(4 * (size_t)std::max<long long>(cap, 1));
+    dt_matching_info info{};
+    if (dt_matching_host(nx, ny, nz, dx, dy, dz, J, epsilon, delta, zeta, (long long)it_max, field, &info,
+                         tr.data(), cap) == 0) {
+        const long long lines = std::min(info.trace_len, cap);
+        for (long long k = 0; k < lines; ++k) print_iteration((long long)tr[4 * k], tr[4 * k + 1], tr[4 * k + 2],
+                                                              tr[4 * k + 3]);
+        std::fflush(stdout);
+    }
 }

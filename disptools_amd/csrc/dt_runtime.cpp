@@ -649,6 +649,48 @@ extern "C" int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz, flo
     return rc;
 }
 
+// The reference's release build prints one line per iteration and a
+// termination message (DISPTOOLS_VERBOSE, setup.py:50; verbose_printf,
+// src/headers/disptools.h:87-91); its entry points here print the same lines
+// from the solve's trace after the solve, unless DISPTOOLS_VERBOSE=0.
+bool dt::verbose_from_env()
+{
+    const char *s = std::getenv("DISPTOOLS_VERBOSE");
+    return !(s && std::strcmp(s, "0") == 0);
+}
+
+void dt::print_iteration(long long it, double E, double m, double eta)
+{
+    std::printf("Iteration %5lld:  total error %6e  max voxel error %6e  eta %6e\n", it, (double)(float)E,
+                (double)(float)m, (double)(float)eta);
+}
+
+// Lines of displacement_field_gradient.c:272-418 / displacement_field_greedy.c:184-236.
+static void print_solve_log(int algorithm, const dt_solve_info &info, const double *tr, long long lines,
+                            long long it_max)
+{
+    for (long long k = 0; k < lines; ++k) {
+        const double *row = tr + 4 * k;
+        print_iteration((long long)row[0], row[1], row[2], row[3]);
+        // greedy: a rejected trial is retried with a smaller step (same iteration)
+        if (algorithm == DT_GREEDY && k + 1 < lines && tr[4 * (k + 1)] == row[0])
+            std::printf("Error not decreasing, reducing step size to %.4e\n", (double)(float)tr[4 * (k + 1) + 3]);
+    }
+    switch (info.reason) {
+    case DT_STOP_ERROR_EXPLODED: std::printf("Terminating: error exploded.\n"); break;
+    case DT_STOP_ETA_SMALL:
+    case DT_STOP_NO_IMPROVEMENT: std::printf("Error not decreasing, terminating.\n"); break;
+    case DT_STOP_VOXEL_EXPLODED: std::printf("Terminating: voxel error exploded.\n"); break;
+    case DT_STOP_TOLERANCE: std::printf("Terminating: reached desired tolerance.\n"); break;
+    default: break;
+    }
+    // printed when the loop counter stopped at it_max (:418), i.e. a stop at
+    // the last iteration; a loop that ran out ends at it_max + 1
+    if (info.reason != DT_STOP_MAX_ITERATIONS && info.iterations == it_max)
+        std::printf("Terminating: reached maximum number of iterations.\n");
+    std::fflush(stdout);
+}
+
 static void reference_entry(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
                             const float *J, const bool *mask, float epsilon, float tolerance, float eta,
                             float eta_max, float alpha, float beta, float gamma, float delta, float zeta,
@@ -659,7 +701,16 @@ static void reference_entry(int algorithm, size_t nx, size_t ny, size_t nz, floa
     p.alpha = alpha; p.beta = beta; p.gamma = gamma; p.delta = delta; p.zeta = zeta;
     p.theta = theta; p.iota = iota; p.strict = strict ? 1 : 0;
     p.it_max = (long long)it_max;
-    (void)dt_solve_host(algorithm, nx, ny, nz, dx, dy, dz, J, mask, &p, field, nullptr, nullptr, 0);
+    if (!verbose_from_env()) {
+        (void)dt_solve_host(algorithm, nx, ny, nz, dx, dy, dz, J, mask, &p, field, nullptr, nullptr, 0);
+        return;
+    }
+    // one line per evaluated trial at most (greedy logs rejected trials too)
+    const long long cap = std::min<long long>(max_passes(p), 1ll << 21);
+    std::vector<double> tr(4 * (size_t)cap);
+    dt_solve_info info{};
+    if (dt_solve_host(algorithm, nx, ny, nz, dx, dy, dz, J, mask, &p, field, &info, tr.data(), cap) == 0)
+        print_solve_log(algorithm, info, tr.data(), std::min(info.trace_len, cap), p.it_max);
 }
 
 extern "C" void generate_displacement_gradient_cuda(
