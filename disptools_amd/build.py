@@ -19,6 +19,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.path.dirname(os.path.dirname(HIPCC))
 ARCH = os.environ.get("DISPTOOLS_OFFLOAD_ARCH", "gfx950")
 
 LIB_NAME = "libdisptools_hip.so"
@@ -90,7 +91,10 @@ def build_library(verbose=True, jobs=None):
         if p.wait() != 0:
             raise subprocess.CalledProcessError(p.returncode, p.args)
     tmp = LIB_PATH + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH] + objs + ["-o", tmp], verbose)
+    # roctx: named ranges per solve for rocprofv3 --marker-trace
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=%s" % ARCH] + objs +
+         ["-L", os.path.join(ROCM, "lib"), "-lrocprofiler-sdk-roctx", "-Wl,-rpath," + os.path.join(ROCM, "lib"),
+          "-o", tmp], verbose)
     os.replace(tmp, LIB_PATH)
     for o in objs:
         os.remove(o)

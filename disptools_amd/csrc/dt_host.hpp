@@ -70,6 +70,14 @@ struct Plan {
     long long n = 0;
 };
 
+// Named range on the calling thread for rocprofv3 --marker-trace (roctx).
+struct MarkerRange {
+    explicit MarkerRange(const char *fmt, ...);
+    ~MarkerRange();
+    MarkerRange(const MarkerRange &) = delete;
+    MarkerRange &operator=(const MarkerRange &) = delete;
+};
+
 // Per-iteration log lines of the reference entry points (DISPTOOLS_VERBOSE).
 bool verbose_from_env();
 void print_iteration(long long it, double E, double m, double eta);

@@ -635,6 +635,7 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
                            dt_matching_info *info, double *trace, long long trace_cap, hipStream_t st)
 {
     const auto t_start = std::chrono::steady_clock::now();
+    const MarkerRange range("dt_matching %dx%dx%d", nx, ny, nz);
     const long long n = (long long)nx * ny * nz;
     MInst h{};
     h.nx = nx; h.ny = ny; h.nz = nz;

@@ -8,6 +8,9 @@
 // src/displacement_field_greedy.c:85-247; the host-driven CUDA loop this
 // replaces is cuda/displacement_field_gradient.cu:164-468.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <cstdarg>
 
 #include <algorithm>
 #include <chrono>
@@ -468,6 +471,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
                         dt_solve_info *info, double *trace, long long trace_cap, hipStream_t st)
 {
     const auto t_start = std::chrono::steady_clock::now();
+    const MarkerRange range("dt_solve %s %dx%dx%d", algorithm == DT_GRADIENT ? "gradient" : "greedy", nx, ny, nz);
     const int precision = precision_from_env();
     const Plan P = make_plan(algorithm, nx, ny, nz, precision, solve_variant(algorithm, nx, ny, nz, dx, dy, dz), 0);
     const long long n = P.n;
@@ -653,6 +657,18 @@ extern "C" int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz, flo
 // termination message (DISPTOOLS_VERBOSE, setup.py:50; verbose_printf,
 // src/headers/disptools.h:87-91); its entry points here print the same lines
 // from the solve's trace after the solve, unless DISPTOOLS_VERBOSE=0.
+dt::MarkerRange::MarkerRange(const char *fmt, ...)
+{
+    char name[128];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(name, sizeof name, fmt, ap);
+    va_end(ap);
+    roctxRangePushA(name);
+}
+
+dt::MarkerRange::~MarkerRange() { roctxRangePop(); }
+
 bool dt::verbose_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_VERBOSE");
