@@ -515,7 +515,9 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipGetDevice(&e->device));
     e->precision = precision_from_env();
     e->variant = (cfg && cfg->variant >= 0 && cfg->variant < GRAD_VARIANTS) ? cfg->variant : variant_from_env();
-    e->greedy_variant = greedy_variant_from_env();
+    // one tile shape per lane: the 32 x 16 fused pass unless overridden
+    const int gv = greedy_variant_from_env();
+    e->greedy_variant = gv >= 0 ? gv : GREEDY_FUSED;
     e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 4;
     e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
     e->tz_policy = cfg ? cfg->tz : 0;

@@ -345,22 +345,32 @@ k_greedy_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
     }
 }
 
+template <int TX, int TY>
 static hipError_t launch_greedy_fused(int precision, const Inst *insts, const Tile *tiles, int nblocks,
                                       hipStream_t st)
 {
     if (precision == 64)
-        hipLaunchKernelGGL((k_greedy_stream<double, GREEDY_TX, GREEDY_TY>), dim3(nblocks),
-                           dim3(GREEDY_TX * GREEDY_TY), 0, st, insts, tiles);
+        hipLaunchKernelGGL((k_greedy_stream<double, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles);
     else
-        hipLaunchKernelGGL((k_greedy_stream<float, GREEDY_TX, GREEDY_TY>), dim3(nblocks),
-                           dim3(GREEDY_TX * GREEDY_TY), 0, st, insts, tiles);
+        hipLaunchKernelGGL((k_greedy_stream<float, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles);
     return hipGetLastError();
+}
+
+template <int TX, int TY>
+static int greedy_fused_blocks_per_cu(int precision)
+{
+    int blocks = 0;
+    const hipError_t r = precision == 64
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_greedy_stream<double, TX, TY>, TX * TY, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_greedy_stream<float, TX, TY>, TX * TY, 0);
+    return (r == hipSuccess && blocks > 0) ? blocks : 1;
 }
 
 hipError_t launch_greedy_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                               hipStream_t st)
 {
-    if (variant == GREEDY_FUSED) return launch_greedy_fused(precision, insts, tiles, nblocks, st);
+    if (variant == GREEDY_FUSED) return launch_greedy_fused<32, 16>(precision, insts, tiles, nblocks, st);
+    if (variant == GREEDY_FUSED_WIDE) return launch_greedy_fused<32, 32>(precision, insts, tiles, nblocks, st);
     hipError_t r = launch_greedy_step(insts, tiles, nblocks, st);
     if (r != hipSuccess) return r;
     return launch_greedy_eval(precision, insts, tiles, nblocks, st);
@@ -368,14 +378,9 @@ hipError_t launch_greedy_pass(int precision, int variant, const Inst *insts, con
 
 int greedy_pass_blocks_per_cu(int precision, int variant)
 {
-    if (variant != GREEDY_FUSED) return 4;
-    int blocks = 0;
-    const hipError_t r = precision == 64
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_greedy_stream<double, GREEDY_TX, GREEDY_TY>,
-                                                       GREEDY_TX * GREEDY_TY, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_greedy_stream<float, GREEDY_TX, GREEDY_TY>,
-                                                       GREEDY_TX * GREEDY_TY, 0);
-    return (r == hipSuccess && blocks > 0) ? blocks : 1;
+    if (variant == GREEDY_FUSED) return greedy_fused_blocks_per_cu<32, 16>(precision);
+    if (variant == GREEDY_FUSED_WIDE) return greedy_fused_blocks_per_cu<32, 32>(precision);
+    return 4;
 }
 
 }  // namespace dt

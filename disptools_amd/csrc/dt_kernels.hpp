@@ -57,15 +57,16 @@ hipError_t launch_smooth_axis(const float *in, float *out, int nx, int ny, int n
 hipError_t launch_resample(const float *in, float *out, int nx, int ny, int nz, int mx, int my, int mz, int ncomp,
                            float rx, float ry, float rz, int nearest, hipStream_t st);
 // Greedy pass variants: 0 = step kernel + evaluation kernel on TILE_X x TILE_Y
-// tiles (dt_kernels.hip); GREEDY_FUSED = one streaming kernel on GREEDY_TX x
-// GREEDY_TY tiles (dt_greedy_stream.hip).
+// tiles (dt_kernels.hip); GREEDY_FUSED = one streaming kernel on 32 x 16
+// tiles (dt_greedy_stream.hip).
+// GREEDY_FUSED_WIDE: the fused kernel on 32 x 32 tiles (16 waves per block).
 constexpr int GREEDY_FUSED = 1;
-constexpr int GREEDY_TX = 32;
-constexpr int GREEDY_TY = 16;
+constexpr int GREEDY_FUSED_WIDE = 2;
+inline bool greedy_fused(int v) { return v == GREEDY_FUSED || v == GREEDY_FUSED_WIDE; }
 inline void greedy_variant_dims(int v, int &tx, int &ty)
 {
-    tx = v == GREEDY_FUSED ? GREEDY_TX : TILE_X;
-    ty = v == GREEDY_FUSED ? GREEDY_TY : TILE_Y;
+    tx = v == GREEDY_FUSED ? 32 : (v == GREEDY_FUSED_WIDE ? 32 : TILE_X);
+    ty = v == GREEDY_FUSED ? 16 : (v == GREEDY_FUSED_WIDE ? 32 : TILE_Y);
 }
 hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);
 hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st);

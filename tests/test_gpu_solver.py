@@ -216,11 +216,12 @@ def test_autotuner_picks_a_timed_variant_and_keeps_parity(gpu_lib, oracle32, ora
     assert abs(mg - mo) <= MAX_ERR_TOL
 
 
-@pytest.mark.parametrize("variant", ["0", "1"])
+@pytest.mark.parametrize("variant", ["0", "1", "2"])
 @pytest.mark.parametrize("precision", ["32", "64"])
 def test_greedy_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant, precision):
-    """The fused greedy pass (1) and the step + evaluation kernels (0), FP32
-    and FP64 stencils, each to the parity bar of test_solver_parity."""
+    """The fused greedy pass on 32x16 (1) and 32x32 (2) tiles and the step +
+    evaluation kernels (0), FP32 and FP64 stencils, each to the parity bar of
+    test_solver_parity."""
     monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", variant)
     monkeypatch.setenv("DISPTOOLS_STENCIL_PRECISION", precision)
     T, M, sp, it_max = CASES["sphere24_aniso"]()
@@ -244,11 +245,16 @@ def test_greedy_first_step_matches_between_variants(gpu_lib, monkeypatch):
     T, M = volume_case((37, 29, 23), 3)
     sp = (1.0, 1.2, 0.9)
     out = {}
-    for v in ("0", "1"):
+    for v in ("0", "1", "2"):
         monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", v)
         print("greedy it_max=1 variant", v, flush=True)
         out[v] = gpu_lib.solve_host("greedy", T, M, sp, it_max=1)
-    (f0, i0, _), (f1, i1, _) = out["0"], out["1"]
-    assert i0["reason"] == i1["reason"] and i0["iterations"] == i1["iterations"]
-    np.testing.assert_allclose(f1, f0, rtol=0, atol=2e-6)
-    assert i1["final_error"] == pytest.approx(i0["final_error"], rel=1e-5)
+    f0, i0, _ = out["0"]
+    for v in ("1", "2"):
+        f1, i1, _ = out[v]
+        assert i0["reason"] == i1["reason"] and i0["iterations"] == i1["iterations"]
+        np.testing.assert_allclose(f1, f0, rtol=0, atol=2e-6)
+        assert i1["final_error"] == pytest.approx(i0["final_error"], rel=1e-5)
+    # the two fused tilings group the FP64 partial sums differently but form
+    # and evaluate the same trial
+    assert np.array_equal(out["1"][0], out["2"][0])
