@@ -45,7 +45,7 @@
 namespace dt {
 namespace {
 
-constexpr int MX = 32, MY = 16, MNT = MX * MY;    // cell tile, one thread per cell
+constexpr int MX = 32, MY = 32, MNT = MX * MY;    // cell tile, one thread per cell
 constexpr int NPX = MX + 1, NPP = NPX * (MY + 1);  // nodes of a cell tile's plane
 constexpr int NEXTRA = NPP - MNT;                  // node points loaded by a second slot
 static_assert(NEXTRA <= MNT, "one extra point per thread at most");
