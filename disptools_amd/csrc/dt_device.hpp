@@ -149,6 +149,16 @@ __device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, double it, doub
     C->trace_len++;
 }
 
+// Completion word of a solve in host-coherent memory: (active passes << 1) |
+// done, stored after every controller step, so the host follows the solve
+// without a copy or an event between pass launches.
+__device__ __forceinline__ void publish_status(unsigned long long *w, long long passes, bool done)
+{
+    if (w)
+        __hip_atomic_store(w, ((unsigned long long)passes << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ void finish(Ctrl *C, int reason)
 {
     C->reason = reason;

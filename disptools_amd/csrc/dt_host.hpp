@@ -24,6 +24,14 @@ std::mutex &solve_lock();
 // Device scratch of at least `bytes` on the current device (grow-only, reused
 // by later calls; valid until the next call under solve_lock()).
 int arena_device_buffer(size_t bytes, void **ptr);
+// Host-coherent completion word of the current device, cleared (one per
+// device, used under solve_lock()): `host` is what the host polls, `dev` what
+// the pass kernels store to.
+int status_word(unsigned long long **host, unsigned long long **dev);
+// Waits until the completion word reports `target` active passes or the end
+// of the solve (`done`).  0: reached; 1: the stream drained without the word
+// reaching the target (the caller reads the control block); -1: device error.
+int follow_status(const unsigned long long *w, long long target, hipStream_t st, bool &done);
 
 #define DT_FAIL(msg)                                         \
     do {                                                     \

@@ -86,6 +86,7 @@ struct Inst {
     Ctrl *ctrl;
     double *trace;           // [trace_cap][4]
     long long trace_cap;
+    unsigned long long *status;   // host-coherent completion word (null: none)
 };
 
 // One entry of a launch's tile table.

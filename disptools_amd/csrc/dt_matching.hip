@@ -81,6 +81,7 @@ struct MInst {
     MCtrl *ctrl;
     double *trace;
     long long trace_cap;
+    unsigned long long *status;   // host-coherent completion word: (passes << 1) | done
 };
 
 // Matrix columns: 0 = difference along y, 1 = along x, 2 = along z; corner
@@ -556,6 +557,9 @@ k_match_pass(const MInst *__restrict__ pinst)
         double sa = 0.0, sb = 0.0;
         for (int k = 0; k < MNT / 64; ++k) { sa += red[2 * k]; sb = fmax(sb, red[2 * k + 1]); }
         match_controller(I, C, sa, (float)sb);
+        const unsigned long long passes = (unsigned long long)(C->cost_passes + C->grad_passes);
+        __hip_atomic_store(I.status, (passes << 1) | (C->phase == MPH_DONE ? 1ull : 0ull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -622,11 +626,6 @@ int resident_match_blocks()
     return device_cus() * per;
 }
 
-struct MatchHost {
-    MCtrl *ctrl = nullptr;     // pinned, 2 slots
-};
-std::vector<MatchHost> g_mhost;
-
 }  // namespace
 
 // Matching solve on device buffers; `field` [3][nz][ny][nx] in/out.
@@ -672,11 +671,8 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     h.trace_cap = trace_cap;
     h.V = d_target;
 
-    int dev = 0;
-    DT_HIP(hipGetDevice(&dev));
-    if ((int)g_mhost.size() <= dev) g_mhost.resize(dev + 1);
-    MatchHost &mh = g_mhost[dev];
-    if (!mh.ctrl) DT_HIP(hipHostMalloc((void **)&mh.ctrl, 2 * sizeof(MCtrl), hipHostMallocDefault));
+    unsigned long long *status = nullptr;
+    if (status_word(&status, &h.status)) return -1;
 
     MCtrl c0{};
     c0.phase = MPH_INIT;
@@ -689,60 +685,61 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     hipLaunchKernelGGL(k_match_in, dim3(grid_of(n)), dim3(256), 0, st, d_field, h.tri[0], nx, ny, nz, dx, dy, dz);
     DT_HIP(hipGetLastError());
 
-    hipEvent_t ev[2] = {nullptr, nullptr}, kev[2][2] = {};
-    for (auto &e : ev) DT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    for (auto &pair : kev)
-        for (auto &e : pair) DT_HIP(hipEventCreate(&e));
-    bool pending[2] = {false, false};
-    double kernel_ms = 0.0;
-    auto account = [&](int s) {
-        float ms = 0.0f;
-        if (pending[s] && hipEventElapsedTime(&ms, kev[s][0], kev[s][1]) == hipSuccess) kernel_ms += ms;
-        pending[s] = false;
-    };
+    // timing events around the back-to-back pass launches
+    hipEvent_t kev[2] = {nullptr, nullptr};
+    for (auto &e : kev) DT_HIP(hipEventCreate(&e));
     // every iteration takes one gradient pass and at most ~160 cost passes
     // (the step doubles from 1 at most 128 times and halves at most 31 times
     // before falling under 1e-9)
     const long long pass_cap = 256 + 192 * (h.it_max + 1);
     long long launched = 0;
     const int chunk = 8;
-    auto enqueue = [&](int slot) -> int {
-        DT_HIP(hipEventRecord(kev[slot][0], st));
+    auto enqueue = [&]() -> int {
         for (int i = 0; i < chunk; ++i) {
             hipLaunchKernelGGL(k_match_pass, dim3(ntiles), dim3(MNT), 0, st, d_inst);
             DT_HIP(hipGetLastError());
         }
         launched += chunk;
-        DT_HIP(hipEventRecord(kev[slot][1], st));
-        pending[slot] = true;
-        DT_HIP(hipMemcpyAsync(&mh.ctrl[slot], h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost, st));
-        DT_HIP(hipEventRecord(ev[slot], st));
         return 0;
     };
-    int rc = enqueue(0);
-    if (!rc) rc = enqueue(1);
-    int slot = 0;
+    // two chunks in flight, the host following the completion word that the
+    // controller stores in host memory (no copy or event between the passes)
+    int rc = hipEventRecord(kev[0], st) == hipSuccess ? 0 : -1;
+    if (!rc) rc = enqueue();
+    long long target = launched;
+    if (!rc) rc = enqueue();
     while (!rc) {
-        if (hipEventSynchronize(ev[slot]) != hipSuccess) {
+        bool done = false;
+        const int w = follow_status(status, target, st, done);
+        if (w < 0) {
             set_error(__LINE__, __FILE__, __func__, "device error during the matching solve");
             rc = -1;
             break;
         }
-        account(slot);
-        if (mh.ctrl[slot].phase == MPH_DONE) break;
+        if (w > 0) {
+            // the stream drained but the word did not follow: read the control block
+            MCtrl c;
+            if (hipMemcpy(&c, h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost) != hipSuccess) {
+                set_error(__LINE__, __FILE__, __func__, "device error during the matching solve");
+                rc = -1;
+                break;
+            }
+            done = c.phase == MPH_DONE;
+        }
+        if (done) break;
         if (launched > pass_cap) {
             set_error(__LINE__, __FILE__, __func__, "matching solve did not terminate");
             rc = -1;
             break;
         }
-        rc = enqueue(slot);
-        slot ^= 1;
+        target = launched;
+        rc = enqueue();
     }
+    const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
     const hipError_t se = hipStreamSynchronize(st);
-    if (se == hipSuccess) { account(0); account(1); }
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    for (auto &pair : kev)
-        for (auto &e : pair) (void)hipEventDestroy(e);
+    float kernel_ms = 0.0f;
+    if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
+    for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 
@@ -766,7 +763,7 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
         info->final_max_error = fin.m;
         info->trace_len = fin.trace_len;
         info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        info->kernel_seconds = kernel_ms * 1e-3;
+        info->kernel_seconds = (double)kernel_ms * 1e-3;
     }
     return 0;
 }

@@ -71,7 +71,8 @@ namespace dt {
 struct Arena {
     void *ptr = nullptr;
     size_t bytes = 0;
-    Ctrl *host_ctrl = nullptr;   // pinned, 2 slots
+    unsigned long long *status = nullptr;       // host-coherent completion word
+    unsigned long long *status_dev = nullptr;   // its device address
 };
 
 static std::mutex g_lock;
@@ -83,7 +84,10 @@ static int arena_get(size_t bytes, Arena **out)
     DT_HIP(hipGetDevice(&dev));
     if ((int)g_arenas.size() <= dev) g_arenas.resize(dev + 1);
     Arena &a = g_arenas[dev];
-    if (!a.host_ctrl) DT_HIP(hipHostMalloc((void **)&a.host_ctrl, 2 * sizeof(Ctrl), hipHostMallocDefault));
+    if (!a.status) {
+        DT_HIP(hipHostMalloc((void **)&a.status, 256, hipHostMallocCoherent));
+        DT_HIP(hipHostGetDevicePointer((void **)&a.status_dev, a.status, 0));
+    }
     if (a.bytes < bytes) {
         if (a.ptr) DT_HIP(hipFree(a.ptr));
         a.ptr = nullptr;
@@ -108,6 +112,44 @@ int arena_device_buffer(size_t bytes, void **ptr)
     if (arena_get(bytes, &A)) return -1;
     *ptr = A->ptr;
     return 0;
+}
+
+int status_word(unsigned long long **host, unsigned long long **dev)
+{
+    Arena *A = nullptr;
+    if (arena_get(0, &A)) return -1;
+    *A->status = 0;
+    *host = A->status;
+    *dev = A->status_dev;
+    return 0;
+}
+
+int follow_status(const unsigned long long *w, long long target, hipStream_t st, bool &done)
+{
+    const volatile unsigned long long *vw = w;
+    auto reached = [&](unsigned long long v) {
+        if (v & 1ull) done = true;
+        return done || (long long)(v >> 1) >= target;
+    };
+    auto last = std::chrono::steady_clock::now();
+    for (unsigned spin = 1;; ++spin) {
+        if (reached(*vw)) return 0;
+        // every few milliseconds: has the device failed, or has the stream
+        // drained without the word reaching the target?
+        if ((spin & 1023u) == 0) {
+            const auto now = std::chrono::steady_clock::now();
+            if (now - last > std::chrono::milliseconds(2)) {
+                last = now;
+                const hipError_t q = hipStreamQuery(st);
+                if (q == hipSuccess) return reached(*vw) ? 0 : 1;
+                if (q != hipErrorNotReady) {
+                    (void)hipGetLastError();
+                    return -1;
+                }
+            }
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -499,9 +541,11 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     const size_t own = scratch_bytes(P, trace_cap, true);
     if (arena_get(own + align_up(sizeof(Inst), 256) + (size_t)ntiles * sizeof(Tile) + 256, &A)) return -1;
     Carver cv(A->ptr);
-    const Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, nullptr, nullptr);
+    Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, nullptr, nullptr);
     Inst *d_inst = cv.take<Inst>(1);
     Tile *d_tiles = cv.take<Tile>((size_t)ntiles);
+    unsigned long long *status = nullptr;
+    if (status_word(&status, &h.status)) return -1;
 
     std::vector<Tile> tiles;
     tiles.reserve(ntiles);
@@ -518,28 +562,16 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         DT_HIP(hipMemsetAsync(h.buf[1], 0, 3 * (size_t)n * sizeof(float), st));
     }
 
-    hipEvent_t ev[2];
-    DT_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    DT_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    // timing events around each chunk's back-to-back pass launches: the
-    // device time of the passes, without the host turnaround between chunks
-    hipEvent_t kev[2][2] = {};
-    for (auto &pair : kev)
-        for (auto &e : pair) DT_HIP(hipEventCreate(&e));
-    bool pending[2] = {false, false};
-    double kernel_ms = 0.0;
-    auto account = [&](int s) {
-        float ms = 0.0f;
-        if (pending[s] && hipEventElapsedTime(&ms, kev[s][0], kev[s][1]) == hipSuccess) kernel_ms += ms;
-        pending[s] = false;
-    };
+    // timing events around the solve's back-to-back pass launches: the device
+    // time of the passes (no host turnaround sits between them)
+    hipEvent_t kev[2] = {nullptr, nullptr};
+    for (auto &e : kev) DT_HIP(hipEventCreate(&e));
 
     // passes needed in the worst case before the controller stops the solve
     const long long pass_cap = max_passes(*p);
     long long launched = 0;
     int rc = 0;
-    auto enqueue = [&](int k, int slot) -> int {
-        DT_HIP(hipEventRecord(kev[slot][0], st));
+    auto enqueue = [&](int k) -> int {
         for (int i = 0; i < k; ++i) {
             if (algorithm == DT_GRADIENT) {
                 DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, st));
@@ -548,41 +580,48 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
             }
         }
         launched += k;
-        DT_HIP(hipEventRecord(kev[slot][1], st));
-        pending[slot] = true;
-        DT_HIP(hipMemcpyAsync(&A->host_ctrl[slot], h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
-        DT_HIP(hipEventRecord(ev[slot], st));
         return 0;
     };
 
-    // small chunks bound the passes that run idle after the solve has stopped
-    // (at most two chunks); two chunks in flight hide the host turnaround
-    int chunk = 4;
-    rc = enqueue(chunk, 0);
-    if (!rc) rc = enqueue(chunk, 1);
-    int slot = 0;
-    bool done = false;
+    // Two chunks in flight; the host follows the completion word the
+    // controller stores in host memory and enqueues the next chunk once the
+    // older one has run, so nothing but pass kernels enters the stream.  Small
+    // chunks bound the passes that run idle after the solve has stopped.
+    const int chunk = 4;
+    rc = hipEventRecord(kev[0], st) == hipSuccess ? 0 : -1;
+    if (!rc) rc = enqueue(chunk);
+    long long target = launched;   // active passes the host waits for
+    if (!rc) rc = enqueue(chunk);
     while (!rc) {
-        if (hipEventSynchronize(ev[slot]) != hipSuccess) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
-        account(slot);
-        if (A->host_ctrl[slot].phase == PH_DONE) { done = true; break; }
+        bool done = false;
+        const int w = follow_status(status, target, st, done);
+        if (w < 0) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
+        if (w > 0) {
+            // the stream drained but the word did not follow: read the control block
+            Ctrl c;
+            if (hipMemcpy(&c, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost) != hipSuccess) {
+                set_error(__LINE__, __FILE__, __func__, "device error during solve");
+                rc = -1;
+                break;
+            }
+            done = c.phase == PH_DONE;
+        }
+        if (done) break;
         if (launched > pass_cap) {
             set_error(__LINE__, __FILE__, __func__, "solver did not terminate (check alpha > 1 and beta < 1)");
             rc = -1;
             break;
         }
-        rc = enqueue(chunk, slot);
-        slot ^= 1;
+        target = launched;
+        rc = enqueue(chunk);
     }
+    const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
     hipError_t se = hipStreamSynchronize(st);
-    if (se == hipSuccess) { account(0); account(1); }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    for (auto &pair : kev)
-        for (auto &e : pair) (void)hipEventDestroy(e);
+    float kernel_ms = 0.0f;
+    if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
+    for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
-    (void)done;
 
     Ctrl fin;
     DT_HIP(hipMemcpy(&fin, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
@@ -606,7 +645,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         info->trace_len = fin.trace_len;
         info->field_io = fin.field_io;
         info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        info->kernel_seconds = kernel_ms * 1e-3;
+        info->kernel_seconds = (double)kernel_ms * 1e-3;
     }
     return 0;
 }
