@@ -152,9 +152,41 @@ k_regularise(float *__restrict__ J, long long n, float eps)
         if (J[i] < eps) J[i] = eps;
 }
 
+// The solve's result into the caller's field (buf[0]) when the controller
+// left it in another buffer (src/displacement_field_gradient.c:421 returns
+// field_[old]).
+__global__ void __launch_bounds__(256)
+k_take_result(const Inst *__restrict__ inst)
+{
+    const Inst &I = *inst;
+    const int rb = __builtin_amdgcn_readfirstlane(I.ctrl->result_buf);
+    if (rb <= 0 || rb >= NBUF) return;
+    const float *__restrict__ src = I.buf[rb];
+    float *__restrict__ dst = I.buf[0];
+    const long long n3 = 3 * I.n;
+    const long long step = (long long)gridDim.x * 256;
+    const long long g = blockIdx.x * 256ll + threadIdx.x;
+    long long done = 0;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+        const long long q = n3 >> 2;
+        for (long long i = g; i < q; i += step) ((float4 *)dst)[i] = ((const float4 *)src)[i];
+        done = q << 2;
+    }
+    for (long long i = done + g; i < n3; i += step) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
+
+hipError_t launch_take_result(const Inst *inst, long long n, hipStream_t st)
+{
+    long long blocks = (3 * n / 4 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_take_result, dim3((unsigned)blocks), dim3(256), 0, st, inst);
+    return hipGetLastError();
+}
 
 hipError_t launch_jacobian(const float *f, float *J, int nx, int ny, int nz, float ix, float iy, float iz,
                            hipStream_t st)

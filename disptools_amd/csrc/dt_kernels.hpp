@@ -73,5 +73,8 @@ hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tile
 hipError_t launch_greedy_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
                               hipStream_t st);
 int greedy_pass_blocks_per_cu(int precision, int variant);
+// Copies buf[ctrl->result_buf] of the instance into buf[0] (nothing when the
+// result already sits there); n voxels.
+hipError_t launch_take_result(const Inst *inst, long long n, hipStream_t st);
 
 }  // namespace dt

@@ -84,6 +84,14 @@ struct MInst {
     unsigned long long *status;   // host-coherent completion word: (passes << 1) | done
 };
 
+// Instance record, control block and arrival counter of a matching solve,
+// staged by one copy.
+struct alignas(256) MHead {
+    MInst inst;
+    MCtrl ctrl;
+    unsigned counter;
+};
+
 // Matrix columns: 0 = difference along y, 1 = along x, 2 = along z; corner
 // bits: 1 = +x, 2 = +y, 4 = +z.
 constexpr int COL_BIT[3] = {2, 1, 4};
@@ -580,10 +588,11 @@ k_match_in(const float *__restrict__ field, float *__restrict__ X, int nx, int n
 
 // node positions -> field (:769-781)
 __global__ void __launch_bounds__(256)
-k_match_out(const float *__restrict__ X, float *__restrict__ field, int nx, int ny, int nz, float dx, float dy,
+k_match_out(const MInst *__restrict__ inst, float *__restrict__ field, int nx, int ny, int nz, float dx, float dy,
             float dz)
 {
     const long long n = (long long)nx * ny * nz;
+    const float *__restrict__ X = inst->tri[__builtin_amdgcn_readfirstlane(inst->ctrl->xb)];
     for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
         const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
         field[v] = (X[n + v] - (float)(i + 1)) * dx;
@@ -591,6 +600,8 @@ k_match_out(const float *__restrict__ X, float *__restrict__ field, int nx, int 
         field[2 * n + v] = (X[2 * n + v] - (float)(k + 1)) * dz;
     }
 }
+
+std::vector<MHead *> g_mstage;   // pinned staging per device
 
 unsigned grid_of(long long work)
 {
@@ -653,9 +664,7 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     for (int t = 0; t < 3; ++t) add(4 * (size_t)n);
     add(12 * (size_t)n);
     add(16 * (size_t)ntiles);
-    add(sizeof(MCtrl));
-    add(sizeof(unsigned));
-    add(sizeof(MInst));
+    add(sizeof(MHead));
     add(32 * (size_t)std::max<long long>(trace_cap, 1));
     void *base = nullptr;
     if (arena_device_buffer(bytes + 4096, &base)) return -1;
@@ -664,9 +673,10 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     for (int t = 0; t < 3; ++t) h.w[t] = cv.take<float>((size_t)n);
     h.D = cv.take<float>(3 * (size_t)n);
     h.partials = cv.take<double>(2 * (size_t)ntiles);
-    h.ctrl = cv.take<MCtrl>(1);
-    h.counter = cv.take<unsigned>(1);
-    MInst *d_inst = cv.take<MInst>(1);
+    MHead *d_head = cv.take<MHead>(1);
+    h.ctrl = &d_head->ctrl;
+    h.counter = &d_head->counter;
+    MInst *d_inst = &d_head->inst;
     h.trace = trace_cap ? cv.take<double>(4 * (size_t)trace_cap) : nullptr;
     h.trace_cap = trace_cap;
     h.V = d_target;
@@ -674,14 +684,21 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     unsigned long long *status = nullptr;
     if (status_word(&status, &h.status)) return -1;
 
-    MCtrl c0{};
-    c0.phase = MPH_INIT;
-    c0.active = 1;
-    c0.xb = 0;
-    c0.alpha = 1.0f;
-    DT_HIP(hipMemcpyAsync(d_inst, &h, sizeof(MInst), hipMemcpyHostToDevice, st));
-    DT_HIP(hipMemcpyAsync(h.ctrl, &c0, sizeof(MCtrl), hipMemcpyHostToDevice, st));
-    DT_HIP(hipMemsetAsync(h.counter, 0, sizeof(unsigned), st));
+    int dev = 0;
+    DT_HIP(hipGetDevice(&dev));
+    if ((int)g_mstage.size() <= dev) g_mstage.resize(dev + 1, nullptr);
+    MHead *&sh = g_mstage[dev];
+    if (!sh) DT_HIP(hipHostMalloc((void **)&sh, sizeof(MHead), hipHostMallocDefault));
+    // one copy from pinned staging (the previous solve on this device has
+    // finished with it: solves end synchronised)
+    sh->inst = h;
+    sh->ctrl = MCtrl{};
+    sh->ctrl.phase = MPH_INIT;
+    sh->ctrl.active = 1;
+    sh->ctrl.xb = 0;
+    sh->ctrl.alpha = 1.0f;
+    sh->counter = 0;
+    DT_HIP(hipMemcpyAsync(d_head, sh, sizeof(MHead), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_match_in, dim3(grid_of(n)), dim3(256), 0, st, d_field, h.tri[0], nx, ny, nz, dx, dy, dz);
     DT_HIP(hipGetLastError());
 
@@ -736,23 +753,26 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
         rc = enqueue();
     }
     const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
-    const hipError_t se = hipStreamSynchronize(st);
+    // positions -> field and the final control block, behind the passes
+    hipError_t se = hipSuccess;
+    if (!rc) {
+        hipLaunchKernelGGL(k_match_out, dim3(grid_of(n)), dim3(256), 0, st, d_inst, d_field, nx, ny, nz, dx, dy, dz);
+        se = hipGetLastError();
+        if (se == hipSuccess) se = hipMemcpyAsync(&sh->ctrl, h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost, st);
+    }
+    const hipError_t ss = hipStreamSynchronize(st);
+    if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
     if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
     for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 
-    MCtrl fin;
-    DT_HIP(hipMemcpy(&fin, h.ctrl, sizeof(MCtrl), hipMemcpyDeviceToHost));
-    hipLaunchKernelGGL(k_match_out, dim3(grid_of(n)), dim3(256), 0, st, h.tri[fin.xb], d_field, nx, ny, nz, dx, dy,
-                       dz);
-    DT_HIP(hipGetLastError());
+    const MCtrl fin = sh->ctrl;
     if (trace && trace_cap > 0) {
         const long long nl = std::min(fin.trace_len, trace_cap);
-        if (nl > 0) DT_HIP(hipMemcpyAsync(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (nl > 0) DT_HIP(hipMemcpy(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost));
     }
-    DT_HIP(hipStreamSynchronize(st));
     if (info) {
         info->iterations = fin.it;
         info->reason = fin.reason;

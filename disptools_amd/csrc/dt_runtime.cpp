@@ -11,6 +11,7 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdarg>
+#include <cstddef>
 
 #include <algorithm>
 #include <chrono>
@@ -73,7 +74,30 @@ struct Arena {
     size_t bytes = 0;
     unsigned long long *status = nullptr;       // host-coherent completion word
     unsigned long long *status_dev = nullptr;   // its device address
+    char *stage = nullptr;                      // pinned staging of a solve's head
+    size_t stage_bytes = 0;
 };
+
+// Instance record, control block and arrival counter of a single-volume
+// solve; its tile table follows in the same allocation, so one copy stages
+// all of them.
+struct alignas(256) SolveHead {
+    Inst inst;
+    Ctrl ctrl;
+    unsigned int counter;
+};
+
+static int arena_stage(Arena &a, size_t bytes)
+{
+    if (a.stage_bytes >= bytes) return 0;
+    if (a.stage) DT_HIP(hipHostFree(a.stage));
+    a.stage = nullptr;
+    a.stage_bytes = 0;
+    const size_t want = align_up(bytes, 4096);
+    DT_HIP(hipHostMalloc((void **)&a.stage, want, hipHostMallocDefault));
+    a.stage_bytes = want;
+    return 0;
+}
 
 static std::mutex g_lock;
 static std::vector<Arena> g_arenas;
@@ -538,24 +562,31 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     if (!trace) trace_cap = 0;
 
     Arena *A = nullptr;
-    const size_t own = scratch_bytes(P, trace_cap, true);
-    if (arena_get(own + align_up(sizeof(Inst), 256) + (size_t)ntiles * sizeof(Tile) + 256, &A)) return -1;
+    const size_t head = sizeof(SolveHead) + (size_t)ntiles * sizeof(Tile);
+    const size_t own = scratch_bytes(P, trace_cap, false);
+    if (arena_get(align_up(head, 256) + own + 256, &A)) return -1;
+    if (arena_stage(*A, head)) return -1;
     Carver cv(A->ptr);
-    Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, nullptr, nullptr);
-    Inst *d_inst = cv.take<Inst>(1);
+    SolveHead *d_head = cv.take<SolveHead>(1);
     Tile *d_tiles = cv.take<Tile>((size_t)ntiles);
+    Inst *d_inst = &d_head->inst;
+    if ((char *)d_tiles != (char *)d_head + sizeof(SolveHead)) DT_FAIL("solve head layout");
+    Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, &d_head->ctrl,
+                       &d_head->counter);
     unsigned long long *status = nullptr;
     if (status_word(&status, &h.status)) return -1;
 
+    // head and tile table in one copy from pinned staging (the previous solve
+    // on this device has finished with the staging: solves end synchronised)
+    SolveHead *sh = (SolveHead *)A->stage;
+    sh->inst = h;
+    sh->ctrl = initial_ctrl(algorithm, *p);
+    sh->counter = 0;
     std::vector<Tile> tiles;
     tiles.reserve(ntiles);
     append_tiles_xcd(tiles, {{0, ntiles}});
-    const Ctrl c = initial_ctrl(algorithm, *p);
-
-    DT_HIP(hipMemcpyAsync(d_inst, &h, sizeof(Inst), hipMemcpyHostToDevice, st));
-    DT_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st));
-    DT_HIP(hipMemcpyAsync(h.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, st));
-    DT_HIP(hipMemsetAsync(h.counter, 0, sizeof(unsigned int), st));
+    std::memcpy(A->stage + sizeof(SolveHead), tiles.data(), tiles.size() * sizeof(Tile));
+    DT_HIP(hipMemcpyAsync(d_head, A->stage, head, hipMemcpyHostToDevice, st));
     if (algorithm == DT_GREEDY) {
         // the second field buffer starts at zero: its outer face is never written
         // by the greedy step (src/disptools.c:61, displacement_field_greedy.c:43-65)
@@ -616,24 +647,23 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         rc = enqueue(chunk);
     }
     const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
-    hipError_t se = hipStreamSynchronize(st);
+    // result into the caller's field and the final control block, behind the passes
+    Ctrl *fin_h = (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
+    hipError_t se = rc ? hipSuccess : launch_take_result(d_inst, n, st);
+    if (se == hipSuccess && !rc) se = hipMemcpyAsync(fin_h, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st);
+    const hipError_t ss = hipStreamSynchronize(st);
+    if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
     if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
     for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 
-    Ctrl fin;
-    DT_HIP(hipMemcpy(&fin, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
-    if (fin.result_buf != 0) {
-        DT_HIP(hipMemcpyAsync(d_field, h.buf[fin.result_buf], 3 * (size_t)n * sizeof(float),
-                              hipMemcpyDeviceToDevice, st));
-    }
+    const Ctrl fin = *fin_h;
     if (trace && trace_cap > 0) {
         const long long nl = std::min(fin.trace_len, trace_cap);
-        if (nl > 0) DT_HIP(hipMemcpyAsync(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (nl > 0) DT_HIP(hipMemcpy(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost));
     }
-    DT_HIP(hipStreamSynchronize(st));
     if (info) {
         info->iterations = fin.it;
         info->evaluations = fin.evaluations;
