@@ -151,12 +151,22 @@ __device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, double it, doub
 
 // Completion word of a solve in host-coherent memory: (active passes << 1) |
 // done, stored after every controller step, so the host follows the solve
-// without a copy or an event between pass launches.
-__device__ __forceinline__ void publish_status(unsigned long long *w, long long passes, bool done)
+// without a copy or an event between pass launches.  Once the solve is done
+// the control block itself is published first (when the instance has a host
+// copy), ordered before the word by a system-scope release.
+__device__ __forceinline__ void publish_status(const Inst &I, const Ctrl *C)
 {
-    if (w)
-        __hip_atomic_store(w, ((unsigned long long)passes << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+    if (!I.status) return;
+    const bool done = C->phase == PH_DONE;
+    if (done && I.final_ctrl) {
+        const unsigned long long *src = (const unsigned long long *)C;
+        unsigned long long *dst = (unsigned long long *)I.final_ctrl;
+        for (int k = 0; k < (int)(sizeof(Ctrl) / 8); ++k)
+            __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+    }
+    __hip_atomic_store(I.status, ((unsigned long long)C->evaluations << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ __forceinline__ void finish(Ctrl *C, int reason)

@@ -10,8 +10,10 @@
 // step and evaluation over all of its; each instance's last block runs that
 // instance's controller, so step lengths, line searches and stopping tests
 // stay per instance and never wait for the host.  Per lane the host enqueues
-// chunks of passes (two in flight), reads back the control blocks once per
-// chunk, retires the instances that have stopped (result copy, slot and
+// chunks of passes (two in flight).  Each controller stores its instance's
+// completion word (and, once done, its control block) in host-coherent
+// memory, so only pass kernels enter a lane's stream: the host follows the
+// words, retires the instances that have stopped (result copy, slot and
 // memory freed) and admits new ones between chunks; the tile table is
 // rebuilt whenever the lane's set of instances changes.  A chunk can be
 // replayed from a captured hipGraph.
@@ -21,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -49,7 +52,6 @@ struct Slot {
     int host_io = 0;
     float *out_field = nullptr;   // where the result goes (caller's buffer)
     size_t pool_off = 0, pool_bytes = 0;
-    long long first_seq = 0;      // first chunk of its lane that includes this instance
     long long launched = 0;       // passes launched over this instance
     int failed = 0;
     Ctrl final_ctrl{};
@@ -59,12 +61,18 @@ struct Slot {
     hipEvent_t ev_done = nullptr;    // result in place
 };
 
+// An instance's share of a chunk: the chunk has run for it once its
+// completion word counts `passes` passes or reports the end of its solve.
+struct Need {
+    int slot;
+    long long ticket;
+    long long passes;
+};
+
 struct Chunk {
     long long seq;
     long long order;  // engine-wide enqueue order
-    int snap;         // pinned snapshot buffer
-    int hi;           // slots [0, hi) were snapshotted
-    hipEvent_t ev;
+    std::vector<Need> needs;
 };
 
 // The instances of one algorithm and their pass pipeline.
@@ -75,14 +83,15 @@ struct Lane {
     bool own_st = true;
     Tile *d_tiles = nullptr;
     size_t d_cap = 0;
-    Tile *h_tiles[2] = {nullptr, nullptr};   // pinned staging per snapshot buffer
+    Tile *h_tiles[2] = {nullptr, nullptr};   // pinned staging, alternating per chunk
+    hipEvent_t ev_tiles[2] = {nullptr, nullptr};   // upload from the staging buffer done
+    bool tiles_pending[2] = {false, false};
     size_t h_cap = 0;
     int ntiles = 0;
     bool dirty = true;
     hipGraphExec_t graph = nullptr;
     int graph_ntiles = -1;
     Tile *graph_tiles = nullptr;
-    Ctrl *h_snap[2] = {nullptr, nullptr};
     std::deque<Chunk> inflight;
     long long seq = 0;
     int n_active = 0;
@@ -163,9 +172,12 @@ struct dt_engine {
     Inst *h_insts = nullptr;     // pinned mirror, one per slot
     Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks
     int *h_zero = nullptr;       // pinned 0 (deactivates a stuck instance)
+    // host-coherent, per slot: completion words, then final control blocks
+    char *coherent = nullptr;
+    char *coherent_dev = nullptr;
+    size_t final_off = 0;
 
     std::vector<Slot> slots;
-    std::vector<hipEvent_t> chunk_events;   // free chunk events
     std::vector<dt_engine_result> results;
     long long order = 0;
     long long next_ticket = 0;
@@ -175,6 +187,9 @@ struct dt_engine {
 
 namespace dt {
 namespace {
+
+volatile unsigned long long *status_of(dt_engine *e, int s) { return (volatile unsigned long long *)e->coherent + s; }
+Ctrl *final_of(dt_engine *e, int s) { return (Ctrl *)(e->coherent + e->final_off) + s; }
 
 int engine_tz(const dt_engine *e, int nz)
 {
@@ -216,6 +231,7 @@ int ensure_tables(Lane &L, size_t need)
     }
     if (L.h_cap < need) {
         DT_HIP(hipStreamSynchronize(L.st));
+        L.tiles_pending[0] = L.tiles_pending[1] = false;
         const size_t cap = std::max<size_t>(need, 2 * L.h_cap);
         for (int s = 0; s < 2; ++s) {
             if (L.h_tiles[s]) DT_HIP(hipHostFree(L.h_tiles[s]));
@@ -227,8 +243,7 @@ int ensure_tables(Lane &L, size_t need)
     return 0;
 }
 
-// Rebuilds and uploads the lane's tile table (staging buffer `stage` is free:
-// the chunk that used it last has completed).
+// Rebuilds and uploads the lane's tile table through staging buffer `stage`.
 int rebuild_tables(dt_engine *e, Lane &L, int stage)
 {
     std::vector<std::pair<int, int>> seg;
@@ -245,8 +260,13 @@ int rebuild_tables(dt_engine *e, Lane &L, int stage)
     append_tiles_xcd(t, seg);
     L.ntiles = (int)t.size();
     if (!t.empty()) {
+        // the staging buffer's previous upload must have been read
+        if (L.tiles_pending[stage]) DT_HIP(hipEventSynchronize(L.ev_tiles[stage]));
+        L.tiles_pending[stage] = false;
         std::memcpy(L.h_tiles[stage], t.data(), t.size() * sizeof(Tile));
         DT_HIP(hipMemcpyAsync(L.d_tiles, L.h_tiles[stage], t.size() * sizeof(Tile), hipMemcpyHostToDevice, L.st));
+        DT_HIP(hipEventRecord(L.ev_tiles[stage], L.st));
+        L.tiles_pending[stage] = true;
     }
     L.dirty = false;
     return 0;
@@ -299,31 +319,16 @@ int enqueue_chunk(dt_engine *e, Lane &L)
     } else {
         if (launch_passes(e, L, L.st)) return -1;
     }
-    int hi = 0;
-    for (int s = 0; s < (int)e->slots.size(); ++s) {
-        Slot &sl = e->slots[s];
-        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
-        hi = s + 1;
-        sl.launched += e->chunk;
-    }
     Chunk c;
     c.seq = L.seq++;
     c.order = e->order++;
-    c.snap = stage;
-    c.hi = hi;
-    if (e->chunk_events.empty()) {
-        hipEvent_t ev;
-        DT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        e->chunk_events.push_back(ev);
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
+        Slot &sl = e->slots[s];
+        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
+        sl.launched += e->chunk;
+        c.needs.push_back({s, sl.ticket, sl.launched});
     }
-    c.ev = e->chunk_events.back();
-    e->chunk_events.pop_back();
-    // the snapshot also copies other lanes' control blocks, which may be in
-    // flux; only this lane's entries are read
-    if (hi > 0)
-        DT_HIP(hipMemcpyAsync(L.h_snap[stage], e->d_ctrl, (size_t)hi * sizeof(Ctrl), hipMemcpyDeviceToHost, L.st));
-    DT_HIP(hipEventRecord(c.ev, L.st));
-    L.inflight.push_back(c);
+    L.inflight.push_back(std::move(c));
     e->stats.chunks++;
     e->stats.passes += e->chunk;
     return 0;
@@ -395,21 +400,91 @@ int reap(dt_engine *e, bool wait)
     return 0;
 }
 
-int process_chunk(dt_engine *e, Lane &L, const Chunk &c)
+// Retires every instance whose completion word reports the end of its solve;
+// returns how many.
+int retire_done(dt_engine *e)
 {
-    const Ctrl *snap = L.h_snap[c.snap];
-    for (int s = 0; s < c.hi; ++s) {
+    int k = 0;
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
         Slot &sl = e->slots[s];
-        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind || c.seq < sl.first_seq) continue;
-        if (snap[s].phase == PH_DONE) {
-            if (retire(e, s, snap[s], 0)) return -1;
-        } else if (sl.launched > max_passes(sl.params)) {
-            // stop its passes, then return the last accepted iterate
-            DT_HIP(hipMemcpyAsync(&e->d_ctrl[s].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
-            Ctrl f = snap[s];
-            f.result_buf = f.base;
-            if (retire(e, s, f, 1)) return -1;
+        if (sl.state != SLOT_ACTIVE || !(*status_of(e, s) & 1ull)) continue;
+        // the device stores the control block before the word
+        std::atomic_thread_fence(std::memory_order_acquire);
+        Ctrl f;
+        std::memcpy(&f, final_of(e, s), sizeof(Ctrl));
+        if (retire(e, s, f, 0)) return -1;
+        ++k;
+    }
+    return k;
+}
+
+bool chunk_ran(dt_engine *e, const Chunk &c)
+{
+    for (const Need &n : c.needs) {
+        const Slot &sl = e->slots[n.slot];
+        if (sl.state != SLOT_ACTIVE || sl.ticket != n.ticket) continue;
+        const unsigned long long v = *status_of(e, n.slot);
+        if ((v & 1ull) || (long long)(v >> 1) >= n.passes) continue;
+        return false;
+    }
+    return true;
+}
+
+// A chunk has run: stop the instances that exceeded their pass budget and
+// return their last accepted iterate.
+int check_stuck(dt_engine *e, Lane &L, const Chunk &c)
+{
+    for (const Need &n : c.needs) {
+        Slot &sl = e->slots[n.slot];
+        if (sl.state != SLOT_ACTIVE || sl.ticket != n.ticket) continue;
+        if (sl.launched <= max_passes(sl.params)) continue;
+        DT_HIP(hipMemcpyAsync(&e->d_ctrl[n.slot].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
+        DT_HIP(hipStreamSynchronize(L.st));
+        Ctrl f;
+        DT_HIP(hipMemcpy(&f, &e->d_ctrl[n.slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+        const int failed = f.phase != PH_DONE;
+        if (failed) f.result_buf = f.base;
+        if (retire(e, n.slot, f, failed)) return -1;
+    }
+    return 0;
+}
+
+// Pops the chunks that have run, in order per lane; returns how many.
+int complete_ready(dt_engine *e)
+{
+    int k = 0;
+    for (Lane &L : e->lanes) {
+        while (!L.inflight.empty() && chunk_ran(e, L.inflight.front())) {
+            const Chunk c = std::move(L.inflight.front());
+            L.inflight.pop_front();
+            if (check_stuck(e, L, c)) return -1;
+            ++k;
         }
+    }
+    return k;
+}
+
+// Device errors; a lane whose stream drained while its words lag behind
+// (host memory not updated by the device) is brought up to date from the
+// control blocks in device memory.
+int check_lanes(dt_engine *e)
+{
+    for (Lane &L : e->lanes) {
+        if (L.inflight.empty()) continue;
+        const hipError_t q = hipStreamQuery(L.st);
+        if (q == hipErrorNotReady) continue;
+        DT_HIP(q);
+        if (chunk_ran(e, L.inflight.front())) continue;
+        for (const Chunk &c : L.inflight)
+            for (const Need &n : c.needs) {
+                const Slot &sl = e->slots[n.slot];
+                if (sl.state != SLOT_ACTIVE || sl.ticket != n.ticket) continue;
+                Ctrl f;
+                DT_HIP(hipMemcpy(&f, &e->d_ctrl[n.slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+                const bool done = f.phase == PH_DONE;
+                if (done) std::memcpy(final_of(e, n.slot), &f, sizeof(Ctrl));
+                *status_of(e, n.slot) = ((unsigned long long)f.evaluations << 1) | (done ? 1ull : 0ull);
+            }
     }
     return 0;
 }
@@ -422,48 +497,35 @@ int fill(dt_engine *e)
     return 0;
 }
 
-int complete(dt_engine *e, Lane &L)
-{
-    const Chunk c = L.inflight.front();
-    L.inflight.pop_front();
-    e->chunk_events.push_back(c.ev);
-    return process_chunk(e, L, c);
-}
-
 int progress(dt_engine *e, int wait)
 {
     if (reap(e, false)) return -1;
+    int moved = retire_done(e);
+    if (moved < 0) return -1;
     if (fill(e)) return -1;
-    // finished chunks of every lane, without blocking
-    bool any = false;
-    for (Lane &L : e->lanes) {
-        while (!L.inflight.empty()) {
-            const hipError_t q = hipEventQuery(L.inflight.front().ev);
-            if (q == hipErrorNotReady) break;
-            DT_HIP(q);
-            if (complete(e, L)) return -1;
-            any = true;
-        }
-    }
-    if (!any && wait) {
-        // wait for whichever lane finishes a chunk first (a blocking wait on
-        // one lane would leave the other idle once its chunks have drained)
+    const int c0 = complete_ready(e);
+    if (c0 < 0) return -1;
+    moved += c0;
+    if (!moved && wait) {
         bool pending = false;
         for (Lane &L : e->lanes) pending |= !L.inflight.empty();
         if (pending) {
-            for (int spin = 0; !any; ++spin) {
-                for (Lane &L : e->lanes) {
-                    if (L.inflight.empty()) continue;
-                    const hipError_t q = hipEventQuery(L.inflight.front().ev);
-                    if (q == hipErrorNotReady) continue;
-                    DT_HIP(q);
-                    if (complete(e, L)) return -1;
-                    any = true;
+            // follow the completion words of every lane (a blocking wait on
+            // one lane would leave the other idle once its chunks have drained)
+            auto last = Clock::now();
+            for (unsigned spin = 1; !moved; ++spin) {
+                const int r = retire_done(e);
+                if (r < 0) return -1;
+                const int c = complete_ready(e);
+                if (c < 0) return -1;
+                moved = r + c;
+                if (moved) break;
+                if ((spin & 63u) == 0 && Clock::now() - last > std::chrono::milliseconds(2)) {
+                    last = Clock::now();
+                    if (check_lanes(e)) return -1;
                 }
-                if (!any) {
-                    if (spin < 64) std::this_thread::yield();
-                    else std::this_thread::sleep_for(std::chrono::microseconds(20));
-                }
+                if (spin < 4096) __builtin_ia32_pause();
+                else std::this_thread::sleep_for(std::chrono::microseconds(10));
             }
         } else if (e->n_retiring > 0) {
             if (reap(e, true)) return -1;
@@ -484,16 +546,14 @@ void destroy(dt_engine *e)
     if (e->io) (void)hipStreamSynchronize(e->io);
     for (Lane &L : e->lanes) {
         if (L.graph) (void)hipGraphExecDestroy(L.graph);
-        for (auto &c : L.inflight) (void)hipEventDestroy(c.ev);
         if (L.d_tiles) (void)hipFree(L.d_tiles);
         for (int s = 0; s < 2; ++s) {
             if (L.h_tiles[s]) (void)hipHostFree(L.h_tiles[s]);
-            if (L.h_snap[s]) (void)hipHostFree(L.h_snap[s]);
+            if (L.ev_tiles[s]) (void)hipEventDestroy(L.ev_tiles[s]);
         }
         if (L.cap) (void)hipStreamDestroy(L.cap);
         if (L.own_st && L.st) (void)hipStreamDestroy(L.st);
     }
-    for (auto ev : e->chunk_events) (void)hipEventDestroy(ev);
     for (auto &sl : e->slots) {
         if (sl.ev_ready) (void)hipEventDestroy(sl.ev_ready);
         if (sl.ev_stop) (void)hipEventDestroy(sl.ev_stop);
@@ -502,6 +562,7 @@ void destroy(dt_engine *e)
     if (e->h_insts) (void)hipHostFree(e->h_insts);
     if (e->h_ctrl0) (void)hipHostFree(e->h_ctrl0);
     if (e->h_zero) (void)hipHostFree(e->h_zero);
+    if (e->coherent) (void)hipHostFree(e->coherent);
     if (e->d_insts) (void)hipFree(e->d_insts);
     if (e->d_ctrl) (void)hipFree(e->d_ctrl);
     if (e->d_counter) (void)hipFree(e->d_counter);
@@ -543,8 +604,7 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
             DT_HIP(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
         }
         DT_HIP(hipStreamCreateWithFlags(&L.cap, hipStreamNonBlocking));
-        for (int s = 0; s < 2; ++s)
-            DT_HIP(hipHostMalloc((void **)&L.h_snap[s], (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
+        for (int s = 0; s < 2; ++s) DT_HIP(hipEventCreateWithFlags(&L.ev_tiles[s], hipEventDisableTiming));
     }
     DT_HIP(hipStreamCreateWithFlags(&e->io, hipStreamNonBlocking));
 
@@ -563,6 +623,10 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
     DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
     *e->h_zero = 0;
+    e->final_off = align_up((size_t)S * sizeof(unsigned long long), 256);
+    DT_HIP(hipHostMalloc((void **)&e->coherent, e->final_off + (size_t)S * sizeof(Ctrl), hipHostMallocCoherent));
+    DT_HIP(hipHostGetDevicePointer((void **)&e->coherent_dev, e->coherent, 0));
+    std::memset(e->coherent, 0, e->final_off + (size_t)S * sizeof(Ctrl));
     e->slots.resize(S);
     for (auto &sl : e->slots) {
         DT_HIP(hipEventCreateWithFlags(&sl.ev_ready, hipEventDisableTiming));
@@ -627,8 +691,11 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
     }
     sl.out_field = field;
     if (!rc) {
+        *status_of(e, s) = 0;
         e->h_insts[s] = make_inst(P, dx, dy, dz, *params, d_target, d_mask, d_field, cv, nullptr, 0,
                                   &e->d_ctrl[s], &e->d_counter[s]);
+        e->h_insts[s].status = (unsigned long long *)e->coherent_dev + s;
+        e->h_insts[s].final_ctrl = (Ctrl *)(e->coherent_dev + e->final_off) + s;
         e->h_ctrl0[s] = initial_ctrl(algorithm, *params);
         if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||
@@ -652,7 +719,6 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
     }
     sl.state = SLOT_ACTIVE;
     sl.ticket = e->next_ticket++;
-    sl.first_seq = L.seq;
     sl.launched = 0;
     e->n_active++;
     L.n_active++;

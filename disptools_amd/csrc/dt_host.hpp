@@ -26,8 +26,10 @@ std::mutex &solve_lock();
 int arena_device_buffer(size_t bytes, void **ptr);
 // Host-coherent completion word of the current device, cleared (one per
 // device, used under solve_lock()): `host` is what the host polls, `dev` what
-// the pass kernels store to.
-int status_word(unsigned long long **host, unsigned long long **dev);
+// the pass kernels store to; `final_*`: where the device publishes the final
+// control block of a gradient or greedy solve.
+int status_word(unsigned long long **host, unsigned long long **dev, Ctrl **final_host = nullptr,
+                Ctrl **final_dev = nullptr);
 // Waits until the completion word reports `target` active passes or the end
 // of the solve (`done`).  0: reached; 1: the stream drained without the word
 // reaching the target (the caller reads the control block); -1: device error.

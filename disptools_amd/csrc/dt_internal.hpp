@@ -67,6 +67,8 @@ struct Ctrl {
     long long field_io;      // 3-component field volumes read + written by evaluated passes
 };
 
+static_assert(sizeof(Ctrl) % 8 == 0, "the control block is published in 8-byte words");
+
 // One solve instance (a volume with its parameters and buffers).
 struct Inst {
     int nx, ny, nz;
@@ -87,6 +89,7 @@ struct Inst {
     double *trace;           // [trace_cap][4]
     long long trace_cap;
     unsigned long long *status;   // host-coherent completion word (null: none)
+    Ctrl *final_ctrl;             // host-coherent copy of the control block once done (null: none)
 };
 
 // One entry of a launch's tile table.

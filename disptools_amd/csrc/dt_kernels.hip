@@ -122,7 +122,7 @@ k_greedy_eval(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
         fold_partials<NT>(I, E, m, G, red);
         if (threadIdx.x == 0) {
             greedy_controller(I, C, E, m, 3);
-            publish_status(I.status, C->evaluations, C->phase == PH_DONE);
+            publish_status(I, C);
             __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }

@@ -72,8 +72,8 @@ namespace dt {
 struct Arena {
     void *ptr = nullptr;
     size_t bytes = 0;
-    unsigned long long *status = nullptr;       // host-coherent completion word
-    unsigned long long *status_dev = nullptr;   // its device address
+    char *coherent = nullptr;       // host-coherent: completion word, final control block
+    char *coherent_dev = nullptr;   // its device address
     char *stage = nullptr;                      // pinned staging of a solve's head
     size_t stage_bytes = 0;
 };
@@ -108,9 +108,9 @@ static int arena_get(size_t bytes, Arena **out)
     DT_HIP(hipGetDevice(&dev));
     if ((int)g_arenas.size() <= dev) g_arenas.resize(dev + 1);
     Arena &a = g_arenas[dev];
-    if (!a.status) {
-        DT_HIP(hipHostMalloc((void **)&a.status, 256, hipHostMallocCoherent));
-        DT_HIP(hipHostGetDevicePointer((void **)&a.status_dev, a.status, 0));
+    if (!a.coherent) {
+        DT_HIP(hipHostMalloc((void **)&a.coherent, 256 + sizeof(Ctrl), hipHostMallocCoherent));
+        DT_HIP(hipHostGetDevicePointer((void **)&a.coherent_dev, a.coherent, 0));
     }
     if (a.bytes < bytes) {
         if (a.ptr) DT_HIP(hipFree(a.ptr));
@@ -138,13 +138,15 @@ int arena_device_buffer(size_t bytes, void **ptr)
     return 0;
 }
 
-int status_word(unsigned long long **host, unsigned long long **dev)
+int status_word(unsigned long long **host, unsigned long long **dev, Ctrl **final_host, Ctrl **final_dev)
 {
     Arena *A = nullptr;
     if (arena_get(0, &A)) return -1;
-    *A->status = 0;
-    *host = A->status;
-    *dev = A->status_dev;
+    *(volatile unsigned long long *)A->coherent = 0;
+    *host = (unsigned long long *)A->coherent;
+    *dev = (unsigned long long *)A->coherent_dev;
+    if (final_host) *final_host = (Ctrl *)(A->coherent + 256);
+    if (final_dev) *final_dev = (Ctrl *)(A->coherent_dev + 256);
     return 0;
 }
 
@@ -574,7 +576,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, &d_head->ctrl,
                        &d_head->counter);
     unsigned long long *status = nullptr;
-    if (status_word(&status, &h.status)) return -1;
+    Ctrl *fin_word = nullptr;   // the control block, published by the device once done
+    if (status_word(&status, &h.status, &fin_word, &h.final_ctrl)) return -1;
 
     // head and tile table in one copy from pinned staging (the previous solve
     // on this device has finished with the staging: solves end synchronised)
@@ -623,10 +626,12 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     if (!rc) rc = enqueue(chunk);
     long long target = launched;   // active passes the host waits for
     if (!rc) rc = enqueue(chunk);
+    bool published = false;   // the end of the solve seen through the completion word
     while (!rc) {
         bool done = false;
         const int w = follow_status(status, target, st, done);
         if (w < 0) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
+        published = w == 0 && done;
         if (w > 0) {
             // the stream drained but the word did not follow: read the control block
             Ctrl c;
@@ -647,10 +652,12 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         rc = enqueue(chunk);
     }
     const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
-    // result into the caller's field and the final control block, behind the passes
-    Ctrl *fin_h = (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
+    // result into the caller's field behind the passes (and the final control
+    // block, unless the device published it)
+    Ctrl *fin_h = published ? fin_word : (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
     hipError_t se = rc ? hipSuccess : launch_take_result(d_inst, n, st);
-    if (se == hipSuccess && !rc) se = hipMemcpyAsync(fin_h, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st);
+    if (se == hipSuccess && !rc && !published)
+        se = hipMemcpyAsync(fin_h, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st);
     const hipError_t ss = hipStreamSynchronize(st);
     if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
@@ -659,7 +666,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 
-    const Ctrl fin = *fin_h;
+    Ctrl fin;
+    std::memcpy(&fin, fin_h, sizeof(Ctrl));
     if (trace && trace_cap > 0) {
         const long long nl = std::min(fin.trace_len, trace_cap);
         if (nl > 0) DT_HIP(hipMemcpy(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost));
