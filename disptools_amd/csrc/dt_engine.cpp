@@ -176,6 +176,8 @@ struct dt_engine {
     char *coherent = nullptr;
     char *coherent_dev = nullptr;
     size_t final_off = 0;
+    bool status_words = true;    // the kernels publish them (DISPTOOLS_STATUS_WORD)
+    std::chrono::steady_clock::time_point last_check{};   // of check_lanes
 
     std::vector<Slot> slots;
     std::vector<dt_engine_result> results;
@@ -497,6 +499,15 @@ int fill(dt_engine *e)
     return 0;
 }
 
+// check_lanes at most every 2 ms while nothing moves
+int maybe_check_lanes(dt_engine *e)
+{
+    const auto now = Clock::now();
+    if (now - e->last_check < std::chrono::milliseconds(2)) return 0;
+    e->last_check = now;
+    return check_lanes(e);
+}
+
 int progress(dt_engine *e, int wait)
 {
     if (reap(e, false)) return -1;
@@ -506,13 +517,15 @@ int progress(dt_engine *e, int wait)
     const int c0 = complete_ready(e);
     if (c0 < 0) return -1;
     moved += c0;
+    if (moved) e->last_check = Clock::now();
+    else if (!wait && maybe_check_lanes(e)) return -1;
     if (!moved && wait) {
         bool pending = false;
         for (Lane &L : e->lanes) pending |= !L.inflight.empty();
         if (pending) {
             // follow the completion words of every lane (a blocking wait on
             // one lane would leave the other idle once its chunks have drained)
-            auto last = Clock::now();
+            e->last_check = Clock::now();
             for (unsigned spin = 1; !moved; ++spin) {
                 const int r = retire_done(e);
                 if (r < 0) return -1;
@@ -520,10 +533,7 @@ int progress(dt_engine *e, int wait)
                 if (c < 0) return -1;
                 moved = r + c;
                 if (moved) break;
-                if ((spin & 63u) == 0 && Clock::now() - last > std::chrono::milliseconds(2)) {
-                    last = Clock::now();
-                    if (check_lanes(e)) return -1;
-                }
+                if ((spin & 63u) == 0 && maybe_check_lanes(e)) return -1;
                 if (spin < 4096) __builtin_ia32_pause();
                 else std::this_thread::sleep_for(std::chrono::microseconds(10));
             }
@@ -581,6 +591,7 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     e->greedy_variant = gv >= 0 ? gv : GREEDY_FUSED;
     e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 4;
     e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
+    e->status_words = status_words_from_env();
     e->tz_policy = cfg ? cfg->tz : 0;
     e->max_slots = (cfg && cfg->max_slots > 0) ? cfg->max_slots : 256;
     if (e->max_slots > MAX_SLOTS) DT_FAIL("max_slots is at most 4096");
@@ -694,8 +705,10 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
         *status_of(e, s) = 0;
         e->h_insts[s] = make_inst(P, dx, dy, dz, *params, d_target, d_mask, d_field, cv, nullptr, 0,
                                   &e->d_ctrl[s], &e->d_counter[s]);
-        e->h_insts[s].status = (unsigned long long *)e->coherent_dev + s;
-        e->h_insts[s].final_ctrl = (Ctrl *)(e->coherent_dev + e->final_off) + s;
+        if (e->status_words) {
+            e->h_insts[s].status = (unsigned long long *)e->coherent_dev + s;
+            e->h_insts[s].final_ctrl = (Ctrl *)(e->coherent_dev + e->final_off) + s;
+        }
         e->h_ctrl0[s] = initial_ctrl(algorithm, *params);
         if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||

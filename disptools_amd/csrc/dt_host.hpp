@@ -101,6 +101,10 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int
 int variant_from_env();
 int greedy_variant_from_env();
 int precision_from_env();
+// DISPTOOLS_STATUS_WORD=0: the kernels publish no completion word; the host
+// follows a solve through its fallback (stream drained -> control block read
+// from device memory).  For testing that path.
+bool status_words_from_env();
 // Tiling of an instance; `variant` is the pass variant of the algorithm
 // (gradient: GRAD_VARIANTS, greedy: 0 or GREEDY_FUSED); tz <= 0 picks the
 // depth for a launch of this instance alone (choose_tz).

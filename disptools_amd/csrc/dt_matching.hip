@@ -566,8 +566,9 @@ k_match_pass(const MInst *__restrict__ pinst)
         for (int k = 0; k < MNT / 64; ++k) { sa += red[2 * k]; sb = fmax(sb, red[2 * k + 1]); }
         match_controller(I, C, sa, (float)sb);
         const unsigned long long passes = (unsigned long long)(C->cost_passes + C->grad_passes);
-        __hip_atomic_store(I.status, (passes << 1) | (C->phase == MPH_DONE ? 1ull : 0ull), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        if (I.status)
+            __hip_atomic_store(I.status, (passes << 1) | (C->phase == MPH_DONE ? 1ull : 0ull), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -683,6 +684,7 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
 
     unsigned long long *status = nullptr;
     if (status_word(&status, &h.status)) return -1;
+    if (!status_words_from_env()) h.status = nullptr;
 
     int dev = 0;
     DT_HIP(hipGetDevice(&dev));
