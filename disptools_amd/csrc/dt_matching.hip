@@ -631,11 +631,32 @@ PassGeom pass_geom(int nx, int ny, int nz, int sx, int sy, int resident)
     return g;
 }
 
+// Per-device state of the matching entry points (used under solve_lock()):
+// the occupancy of the pass kernel (queried once) and the timing events.
+struct MatchDevice {
+    int per_cu = 0;
+    hipEvent_t kev[2] = {nullptr, nullptr};
+};
+std::vector<MatchDevice> g_mdev;
+
+MatchDevice &match_device_state()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if ((int)g_mdev.size() <= dev) g_mdev.resize(dev + 1);
+    return g_mdev[dev];
+}
+
 int resident_match_blocks()
 {
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_match_pass, MNT, 0) != hipSuccess || per < 1) per = 1;
-    return device_cus() * per;
+    MatchDevice &md = match_device_state();
+    if (md.per_cu < 1) {
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_match_pass, MNT, 0) != hipSuccess || per < 1)
+            per = 1;
+        md.per_cu = per;
+    }
+    return device_cus() * md.per_cu;
 }
 
 }  // namespace
@@ -705,8 +726,10 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     DT_HIP(hipGetLastError());
 
     // timing events around the back-to-back pass launches
-    hipEvent_t kev[2] = {nullptr, nullptr};
-    for (auto &e : kev) DT_HIP(hipEventCreate(&e));
+    MatchDevice &md = match_device_state();
+    for (auto &e : md.kev)
+        if (!e) DT_HIP(hipEventCreate(&e));
+    hipEvent_t *kev = md.kev;
     // every iteration takes one gradient pass and at most ~160 cost passes
     // (the step doubles from 1 at most 128 times and halves at most 31 times
     // before falling under 1e-9)
@@ -766,7 +789,6 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
     if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
-    for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 

@@ -76,6 +76,7 @@ struct Arena {
     char *coherent_dev = nullptr;   // its device address
     char *stage = nullptr;                      // pinned staging of a solve's head
     size_t stage_bytes = 0;
+    hipEvent_t kev[2] = {nullptr, nullptr};     // timing events of a solve's passes
 };
 
 // Instance record, control block and arrival counter of a single-volume
@@ -108,6 +109,10 @@ static int arena_get(size_t bytes, Arena **out)
     DT_HIP(hipGetDevice(&dev));
     if ((int)g_arenas.size() <= dev) g_arenas.resize(dev + 1);
     Arena &a = g_arenas[dev];
+    if (!a.kev[0]) {
+        DT_HIP(hipEventCreate(&a.kev[0]));
+        DT_HIP(hipEventCreate(&a.kev[1]));
+    }
     if (!a.coherent) {
         DT_HIP(hipHostMalloc((void **)&a.coherent, 256 + sizeof(Ctrl), hipHostMallocCoherent));
         DT_HIP(hipHostGetDevicePointer((void **)&a.coherent_dev, a.coherent, 0));
@@ -262,15 +267,40 @@ Ctrl initial_ctrl(int algorithm, const dt_params &p)
     return c;
 }
 
+// Per-device values that do not change while the process runs (queried once:
+// hipGetDeviceProperties and the occupancy calculator cost tens of
+// microseconds, which a small solve would pay on every call).
+static std::mutex g_query_lock;
+static std::map<std::tuple<int, int, int, int>, int> g_queries;   // (device, kind, a, b) -> value
+
+template <typename F>
+static int cached_query(int kind, int a, int b, F query)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return query();
+    const auto key = std::make_tuple(dev, kind, a, b);
+    {
+        std::lock_guard<std::mutex> g(g_query_lock);
+        auto it = g_queries.find(key);
+        if (it != g_queries.end()) return it->second;
+    }
+    const int v = query();
+    std::lock_guard<std::mutex> g(g_query_lock);
+    g_queries[key] = v;
+    return v;
+}
+
 int device_cus()
 {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-            cus = prop.multiProcessorCount;
-    }
-    return cus;
+    return cached_query(0, 0, 0, [] {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+                cus = v;
+        }
+        return cus;
+    });
 }
 
 // A block streams TZ planes plus a pipeline of `overhead` plane steps.
@@ -361,8 +391,10 @@ static int plan_tz(const Plan &P)
     const int cus = device_cus();
     const bool fused_greedy = P.algorithm == DT_GREEDY && greedy_fused(P.variant);
     const bool stream = fused_greedy || (P.algorithm == DT_GRADIENT && P.variant >= STREAM_BASE);
-    const int per_cu = P.algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(P.precision, P.variant)
-                                                  : greedy_pass_blocks_per_cu(P.precision, P.variant);
+    const int per_cu = cached_query(1 + P.algorithm, P.precision, P.variant, [&] {
+        return P.algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(P.precision, P.variant)
+                                          : greedy_pass_blocks_per_cu(P.precision, P.variant);
+    });
     return choose_tz(P.nx, P.ny, P.nz, P.tx, P.ty, cus, cus * per_cu, fused_greedy ? 4 : 3, stream);
 }
 
@@ -605,8 +637,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
 
     // timing events around the solve's back-to-back pass launches: the device
     // time of the passes (no host turnaround sits between them)
-    hipEvent_t kev[2] = {nullptr, nullptr};
-    for (auto &e : kev) DT_HIP(hipEventCreate(&e));
+    hipEvent_t *kev = A->kev;
 
     // passes needed in the worst case before the controller stops the solve
     const long long pass_cap = max_passes(*p);
@@ -669,7 +700,6 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
     if (se != hipSuccess || !timed || hipEventElapsedTime(&kernel_ms, kev[0], kev[1]) != hipSuccess) kernel_ms = 0.0f;
-    for (auto &e : kev) (void)hipEventDestroy(e);
     if (se != hipSuccess && !rc) { set_error(__LINE__, __FILE__, __func__, hipGetErrorString(se)); rc = -1; }
     if (rc) return -1;
 
