@@ -21,6 +21,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -179,7 +180,9 @@ int follow_status(const unsigned long long *w, long long target, hipStream_t st,
                 }
             }
         }
-        __builtin_ia32_pause();
+        // spin while a chunk is short; long waits give the core away
+        if (spin < 65536) __builtin_ia32_pause();
+        else std::this_thread::yield();
     }
 }
 
