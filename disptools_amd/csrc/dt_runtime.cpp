@@ -1,10 +1,10 @@
 // dt_runtime.cpp -- host runtime and C ABI of the HIP displacement library.
 //
 // A solve keeps all of its state on the device.  The host enqueues chunks of
-// pass kernels and reads back only the 200-byte control block once per chunk
-// (two chunks in flight), so the line search and the stopping tests never
-// wait for the host; passes launched after the solve has stopped return at
-// once.  Reference behaviour: src/displacement_field_gradient.c:191-431,
+// pass kernels (two chunks in flight) and follows the completion word the
+// pass controller stores in host-coherent memory, so the line search and the
+// stopping tests never wait for the host and nothing but pass kernels enters
+// the stream; passes launched after the solve has stopped return at once.  Reference behaviour: src/displacement_field_gradient.c:191-431,
 // src/displacement_field_greedy.c:85-247; the host-driven CUDA loop this
 // replaces is cuda/displacement_field_gradient.cu:164-468.
 #include <hip/hip_runtime.h>
