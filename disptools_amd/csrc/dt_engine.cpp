@@ -635,7 +635,7 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
     *e->h_zero = 0;
     e->final_off = align_up((size_t)S * sizeof(unsigned long long), 256);
-    DT_HIP(hipHostMalloc((void **)&e->coherent, e->final_off + (size_t)S * sizeof(Ctrl), hipHostMallocCoherent));
+    DT_HIP(host_words_alloc((void **)&e->coherent, e->final_off + (size_t)S * sizeof(Ctrl)));
     DT_HIP(hipHostGetDevicePointer((void **)&e->coherent_dev, e->coherent, 0));
     std::memset(e->coherent, 0, e->final_off + (size_t)S * sizeof(Ctrl));
     e->slots.resize(S);

@@ -34,6 +34,8 @@ int status_word(unsigned long long **host, unsigned long long **dev, Ctrl **fina
 // of the solve (`done`).  0: reached; 1: the stream drained without the word
 // reaching the target (the caller reads the control block); -1: device error.
 int follow_status(const unsigned long long *w, long long target, hipStream_t st, bool &done);
+// Pinned host memory for completion words and published control blocks.
+hipError_t host_words_alloc(void **p, size_t bytes);
 
 #define DT_FAIL(msg)                                         \
     do {                                                     \

@@ -104,6 +104,16 @@ static int arena_stage(Arena &a, size_t bytes)
 static std::mutex g_lock;
 static std::vector<Arena> g_arenas;
 
+hipError_t host_words_alloc(void **p, size_t bytes)
+{
+    // host-coherent where the platform offers it; otherwise plain pinned
+    // memory, where the device's stores may only land at the end of a kernel
+    // (the host then follows a solve through its drained-stream fallback)
+    if (hipHostMalloc(p, bytes, hipHostMallocCoherent) == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();
+    return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+
 static int arena_get(size_t bytes, Arena **out)
 {
     int dev = 0;
@@ -115,7 +125,7 @@ static int arena_get(size_t bytes, Arena **out)
         DT_HIP(hipEventCreate(&a.kev[1]));
     }
     if (!a.coherent) {
-        DT_HIP(hipHostMalloc((void **)&a.coherent, 256 + sizeof(Ctrl), hipHostMallocCoherent));
+        DT_HIP(host_words_alloc((void **)&a.coherent, 256 + sizeof(Ctrl)));
         DT_HIP(hipHostGetDevicePointer((void **)&a.coherent_dev, a.coherent, 0));
     }
     if (a.bytes < bytes) {
