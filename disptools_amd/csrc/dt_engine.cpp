@@ -534,7 +534,7 @@ int progress(dt_engine *e, int wait)
                 moved = r + c;
                 if (moved) break;
                 if ((spin & 63u) == 0 && maybe_check_lanes(e)) return -1;
-                if (spin < 4096) __builtin_ia32_pause();
+                if (spin < 4096) cpu_relax();
                 else std::this_thread::sleep_for(std::chrono::microseconds(10));
             }
         } else if (e->n_retiring > 0) {

@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "disptools_hip.h"
@@ -36,6 +37,16 @@ int status_word(unsigned long long **host, unsigned long long **dev, Ctrl **fina
 int follow_status(const unsigned long long *w, long long target, hipStream_t st, bool &done);
 // Pinned host memory for completion words and published control blocks.
 hipError_t host_words_alloc(void **p, size_t bytes);
+
+// One iteration of a host spin-wait.
+inline void cpu_relax()
+{
+#if defined(__x86_64__) || defined(__i386__)
+    __builtin_ia32_pause();
+#else
+    std::this_thread::yield();
+#endif
+}
 
 #define DT_FAIL(msg)                                         \
     do {                                                     \

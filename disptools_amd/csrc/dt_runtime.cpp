@@ -191,7 +191,7 @@ int follow_status(const unsigned long long *w, long long target, hipStream_t st,
             }
         }
         // spin while a chunk is short; long waits give the core away
-        if (spin < 65536) __builtin_ia32_pause();
+        if (spin < 65536) cpu_relax();
         else std::this_thread::yield();
     }
 }
