@@ -156,7 +156,7 @@ struct dt_engine {
     int precision = 32;
     int variant = 1;             // gradient pass variant
     int greedy_variant = 1;      // greedy pass variant
-    int chunk = 4;
+    int chunk = 2;
     int use_graphs = 0;
     int tz_policy = 0;
     int max_slots = 256;
@@ -589,7 +589,10 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     // one tile shape per lane: the 32 x 16 fused pass unless overridden
     const int gv = greedy_variant_from_env();
     e->greedy_variant = gv >= 0 ? gv : GREEDY_FUSED;
-    e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 4;
+    // short chunks: no host round trip separates them, and an admitted
+    // instance joins at the next chunk (64 volumes U{48..160}^3: 2 passes per
+    // chunk: 1 % faster than 4, 3.5 % than 8; serving percentiles unchanged)
+    e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 2;
     e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
     e->status_words = status_words_from_env();
     e->tz_policy = cfg ? cfg->tz : 0;

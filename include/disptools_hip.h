@@ -121,8 +121,8 @@ typedef struct dt_solve_info {
     long long field_io;      /* [3][nz][ny][nx] field volumes read+written by
                                 the evaluated passes (algorithmic traffic)  */
     double kernel_seconds;   /* device time of the pass launches (events
-                                around each chunk of back-to-back launches;
-                                0 in engine results)                        */
+                                before the first and after the last of the
+                                back-to-back launches; 0 in engine results) */
 } dt_solve_info;
 
 /* Record of a matching solve (dt_matching_device / dt_matching_host). */
@@ -237,7 +237,7 @@ typedef struct dt_engine dt_engine;
 typedef struct dt_engine_config {
     size_t hbm_budget;   /* device bytes for instance state (0: half of the free memory) */
     int max_slots;       /* instances on the device at once (0: 256; at most 4096)       */
-    int chunk;           /* passes enqueued per chunk (0: 4)                               */
+    int chunk;           /* passes enqueued per chunk (0: 2)                               */
     int use_graphs;      /* 1: each chunk is replayed from a captured hipGraph             */
     int variant;         /* gradient pass variant (-1: the library default)                */
     int tz;              /* tile depth in planes: 0 = batch default (balanced, <= 16),
@@ -287,7 +287,8 @@ long long dt_engine_submit(dt_engine *e, int algorithm, size_t nx, size_t ny, si
                            const dt_params *params, float *field, int host_io);
 
 /* Advances the engine: keeps two chunks of passes in flight, retires
- * finished instances.  wait = 1 blocks until the oldest chunk has finished.
+ * finished instances.  wait = 1 blocks until a chunk has run or an instance
+ * has finished.
  * Returns the number of instances not yet reported, or -1 on error. */
 int dt_engine_progress(dt_engine *e, int wait);
 
