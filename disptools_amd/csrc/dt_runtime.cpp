@@ -671,8 +671,9 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     // Two chunks in flight; the host follows the completion word the
     // controller stores in host memory and enqueues the next chunk once the
     // older one has run, so nothing but pass kernels enters the stream.  Small
-    // chunks bound the passes that run idle after the solve has stopped.
-    const int chunk = 4;
+    // chunks bound the passes that run idle after the solve has stopped (128^3
+    // gradient solve: 2 idle launches with 2-pass chunks, 4 with 4-pass ones).
+    const int chunk = 2;
     rc = hipEventRecord(kev[0], st) == hipSuccess ? 0 : -1;
     if (!rc) rc = enqueue(chunk);
     long long target = launched;   // active passes the host waits for
