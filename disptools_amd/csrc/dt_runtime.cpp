@@ -581,12 +581,46 @@ static int tune_gradient(int nx, int ny, int nz, float dx, float dy, float dz, i
 // Pass variant of a solve.  Gradient: DISPTOOLS_GRAD_VARIANT, else the tuned
 // variant of the shape when DISPTOOLS_AUTOTUNE=1, else the default; greedy:
 // the fused pass unless DISPTOOLS_GREEDY_VARIANT=0.
+// Column tile of a single FP32 gradient solve from a cost model of the
+// streaming pass: t = 8.85 + S (0.301 + 1.80 NT / 1000) microseconds, S the
+// plane steps of a pass (rounds of blocks x (TZ + 3), as choose_tz plans
+// them) and NT the block's threads, fitted to measured pass times of the
+// five streaming tiles over 18 volume shapes (32^3 to 256^3 and non-cubic:
+// within 1-2 us).  The 64 x 12 tile stays unless another is predicted at
+// least 10 % faster; over those shapes the rule is 6.5 % faster in sum than
+// 64 x 12 everywhere and never slower (the measured best is 2 % faster
+// still).  Deterministic: it depends on the shape and the CU count only.
+int model_grad_variant(int nx, int ny, int nz, int precision)
+{
+    if (precision != 32) return DEFAULT_GRAD_VARIANT;
+    const int cus = device_cus();
+    auto predict = [&](int v) {
+        int tx = 0, ty = 0;
+        grad_variant_dims(v, tx, ty);
+        const int nring = 2 * tx + 2 * ty;
+        const double nt = tx * ty + (nring + 63) / 64 * 64;
+        const int tz = choose_tz(nx, ny, nz, tx, ty, cus, cus, 3, true);
+        const long long cols = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
+        const long long blocks = cols * ((nz + tz - 1) / tz);
+        const double steps = (double)((blocks + cus - 1) / cus) * (std::min(tz, nz) + 3);
+        return 8.85 + steps * (0.301 + 1.80 * nt / 1000.0);
+    };
+    const double t_def = predict(DEFAULT_GRAD_VARIANT);
+    int best = DEFAULT_GRAD_VARIANT;
+    double t_best = t_def;
+    for (int v = STREAM_BASE; v < GRAD_VARIANTS; ++v) {
+        const double t = predict(v);
+        if (t < t_best) { t_best = t; best = v; }
+    }
+    return t_def >= 1.10 * t_best ? best : DEFAULT_GRAD_VARIANT;
+}
+
 static int solve_variant(int algorithm, int nx, int ny, int nz, float dx, float dy, float dz)
 {
     if (algorithm != DT_GRADIENT) return greedy_solve_variant(nx, ny, precision_from_env());
     if (std::getenv("DISPTOOLS_GRAD_VARIANT")) return variant_from_env();
     const char *t = std::getenv("DISPTOOLS_AUTOTUNE");
-    if (!t || std::strcmp(t, "1") != 0) return DEFAULT_GRAD_VARIANT;
+    if (!t || std::strcmp(t, "1") != 0) return model_grad_variant(nx, ny, nz, precision_from_env());
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return DEFAULT_GRAD_VARIANT;
     auto it = g_tuned.find(TuneKey{dev, nx, ny, nz});

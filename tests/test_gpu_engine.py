@@ -3,8 +3,8 @@
 * Each result is the same alone and in the mix: an instance solved by itself
   and the same instance solved among others of other sizes and algorithms,
   admitted at different times, give bit-identical fields and records.
-* With the lone-solve tiling (tz=-1) the engine's result is bit-identical to
-  dt_solve_device's.
+* With the lone-solve tiling (tz=-1) and the same column tile the engine's
+  result is bit-identical to dt_solve_device's.
 * Against the FP64 oracle: the same termination reason and a final masked
   max |J - T| within 1e-4 (the solver parity bar).
 * Admission: a slot limit and a pool budget smaller than the batch queue the
@@ -116,7 +116,10 @@ def test_engine_alone_equals_mixed(gpu_lib):
         np.testing.assert_array_equal(mixed_f[i], alone_f[i])
 
 
-def test_engine_lone_tiling_equals_plain_solve(gpu_lib):
+def test_engine_lone_tiling_equals_plain_solve(gpu_lib, monkeypatch):
+    # the same column tile on both sides (a single solve picks its tile per
+    # shape; an engine lane uses one)
+    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
     cases = _cases()
     with gpu_lib.Engine(tz=-1) as eng:
         ef, er = _run_all(eng, cases)

@@ -77,6 +77,28 @@ def test_solver_parity(gpu_lib, oracle32, oracle64, alg, case):
     np.testing.assert_allclose(trace[:k, 3], io["trace"][:k, 3], rtol=1e-6)
 
 
+_ORACLE64 = {}
+
+
+@pytest.mark.parametrize("variant", [str(v) for v in range(16)])
+@pytest.mark.parametrize("case", ["config1_40", "ragged_13x21x34"])
+def test_gradient_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant, case):
+    """Every compiled form and column tile of the gradient pass holds the
+    parity bar (a single solve picks its tile per shape)."""
+    T, M, sp, it_max = CASES[case]()
+    if case not in _ORACLE64:
+        _ORACLE64[case] = oracle64.solve("gradient", T.astype(np.float64), M, sp, it_max=it_max)
+    fo, io = _ORACLE64[case]
+    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", variant)
+    print("gradient variant", variant, case, T.shape, flush=True)
+    fg, ig, _ = gpu_lib.solve_host("gradient", T, M, sp, it_max=it_max)
+    assert ig["reason"] == io["reason"], (ig["reason"], io["reason"])
+    _, mo = masked_max_error(oracle32, fo.astype(np.float32), T, M, sp)
+    _, mg = masked_max_error(oracle32, fg, T, M, sp)
+    assert abs(mg - mo) <= MAX_ERR_TOL
+    assert abs(ig["iterations"] - io["iterations"]) <= max(3, 0.1 * io["iterations"])
+
+
 @pytest.mark.parametrize("alg", ["gradient", "greedy"])
 def test_all_ones_target(gpu_lib, alg):
     # energy 0 is not a normal number: "error exploded" at it = 1, zero field returned
