@@ -338,7 +338,9 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int
 
 // Default pass variant: the single-barrier streaming pass on 64 x 12 column
 // tiles with the tile ring in waves of its own (measured fastest at 128^3
-// among the compiled variants); DISPTOOLS_GRAD_VARIANT overrides it.
+// among the compiled variants).  Engine lanes use it; single solves start
+// from it and switch tile by model_grad_variant; DISPTOOLS_GRAD_VARIANT
+// overrides both.
 constexpr int DEFAULT_GRAD_VARIANT = STREAM_BASE + 4;
 
 int variant_from_env()
