@@ -10,8 +10,11 @@ Both travel as bytes:
     raw arrays, little-endian         request: target f4, mask u1, [field f4]
                                       response: field f4
 
-``SolveServer`` admits decoded requests into a continuous-batching engine as
-slots and device memory free up and hands back responses as solves finish.
+``SolveServer`` admits decoded gradient and greedy requests into a
+continuous-batching engine as slots and device memory free up and hands back
+responses as solves finish; matching requests (the mask is carried and, as
+in the reference, unused) are solved on the device one at a time between
+engine steps.
 """
 
 import json
@@ -23,8 +26,11 @@ import numpy as np
 REQUEST_MAGIC = b"DTRQ1\n"
 RESPONSE_MAGIC = b"DTRS1\n"
 
+ALGORITHMS = ("gradient", "greedy", "matching")
+
 # Parameters a request may set (the reference's solver arguments,
-# displacements.py:246-260, plus it_max).
+# displacements.py:246-260, plus it_max; matching reads epsilon, delta, zeta
+# and it_max).
 PARAM_NAMES = ("epsilon", "tolerance", "eta", "eta_max", "alpha", "beta", "gamma", "delta", "zeta",
                "theta", "iota", "strict", "it_max")
 
@@ -83,8 +89,8 @@ def encode_request(req: SolveRequest) -> bytes:
     unknown = set(req.params) - set(PARAM_NAMES)
     if unknown:
         raise ValueError("unknown solver parameters: %s" % sorted(unknown))
-    if req.algorithm not in ("gradient", "greedy"):
-        raise ValueError("algorithm must be 'gradient' or 'greedy'")
+    if req.algorithm not in ALGORITHMS:
+        raise ValueError("algorithm must be one of %s" % (ALGORITHMS,))
     f = None
     if req.field is not None:
         f = np.ascontiguousarray(req.field, "<f4")
@@ -116,6 +122,8 @@ def decode_request(buf) -> SolveRequest:
     unknown = set(params) - set(PARAM_NAMES)
     if unknown:
         raise ValueError("unknown solver parameters: %s" % sorted(unknown))
+    if head.get("algorithm") not in ALGORITHMS:
+        raise ValueError("unknown algorithm %r" % (head.get("algorithm"),))
     return SolveRequest(target=t.astype(np.float32), mask=m.astype(bool), spacing=tuple(head["spacing"]),
                         algorithm=head["algorithm"], params=params,
                         field=None if f is None else f.astype(np.float32), request_id=head["id"])
@@ -148,7 +156,9 @@ class SolveServer:
 
     ``submit`` queues a request; ``step`` admits queued requests into free
     slots (the inputs are staged from pinned host buffers), advances the
-    passes and returns the responses of the solves that finished.
+    passes and returns the responses of the solves that finished.  Matching
+    requests wait in a queue of their own; ``step`` solves the oldest one on
+    the device (the engine's passes already in flight keep running meanwhile).
     """
 
     def __init__(self, hbm_budget=0, max_slots=0, **engine_kw):
@@ -158,6 +168,7 @@ class SolveServer:
         self._native = native
         self.engine = native.Engine(hbm_budget=hbm_budget, max_slots=max_slots, **engine_kw)
         self._queue: List[Tuple[SolveRequest, tuple]] = []
+        self._matching: List[SolveRequest] = []
         self._live: Dict[int, Tuple[SolveRequest, tuple]] = {}
 
     def close(self):
@@ -170,9 +181,14 @@ class SolveServer:
         self.close()
 
     def pending(self):
-        return len(self._queue) + len(self._live)
+        return len(self._queue) + len(self._matching) + len(self._live)
 
     def submit(self, req: SolveRequest):
+        if req.algorithm not in ALGORITHMS:
+            raise ValueError("algorithm must be one of %s" % (ALGORITHMS,))
+        if req.algorithm == "matching":
+            self._matching.append(req)
+            return
         torch = self._torch
         t = torch.from_numpy(np.ascontiguousarray(req.target, np.float32)).pin_memory()
         m = torch.from_numpy(np.ascontiguousarray(req.mask, np.uint8)).pin_memory()
@@ -193,11 +209,35 @@ class SolveServer:
                 return
             self._live[ticket] = self._queue.pop(0)
 
+    def _solve_matching(self, req: SolveRequest) -> SolveResponse:
+        import time
+        t0 = time.perf_counter()
+        keys = ("epsilon", "delta", "zeta", "it_max")
+        kw = {k: req.params[k] for k in keys if k in req.params}
+        status, info = 0, None
+        try:
+            f, info, _ = self._native.matching_host(req.target, req.spacing, field=req.field, **kw)
+        except self._native.HipError:
+            status = -1
+            f = (np.zeros((3,) + tuple(req.target.shape), np.float32) if req.field is None
+                 else np.array(req.field, np.float32, copy=True))
+        if info is None:
+            return SolveResponse(request_id=req.request_id, status=status, field=f,
+                                 seconds=time.perf_counter() - t0)
+        return SolveResponse(request_id=req.request_id, status=status, field=f, reason=info["reason"],
+                             reason_text=self._native.REASONS[info["reason"]], iterations=info["iterations"],
+                             evaluations=info["cost_passes"] + info["gradient_passes"], rejections=0,
+                             final_error=info["final_cost"], final_max_error=info["final_max_error"],
+                             seconds=time.perf_counter() - t0)
+
     def step(self, wait=True) -> List[SolveResponse]:
         self._admit()
+        out = []
+        if self._matching:
+            out.append(self._solve_matching(self._matching.pop(0)))
+            wait = False
         if self._live:
             self.engine.progress(wait=wait)
-        out = []
         for ticket, status, info in self.engine.poll():
             req, (t, m, f) = self._live.pop(ticket)
             out.append(SolveResponse(request_id=req.request_id, status=status, field=f.numpy().copy(),

@@ -31,3 +31,23 @@ def test_server_handles_encoded_requests(gpu_lib, oracle32, oracle64):
         _, mg, _ = oracle32.compute_error(r.target, oracle32.jacobian(res.field, r.spacing), r.mask, 0.2)
         print(r.request_id, r.algorithm, "max error: server %.7g  fp64 oracle %.7g" % (mg, mo))
         assert abs(mg - mo) <= 1e-4
+
+
+def test_server_solves_matching_requests(gpu_lib):
+    """Matching requests beside engine requests: the response equals the
+    matching entry point's result for the same input, bit for bit."""
+    from conftest import sphere
+    from disptools_amd.serving import SolveRequest, SolveServer, decode_response, encode_request
+    T1, M1 = sphere(16, 5)
+    T2, M2 = sphere(20, 6, inside=0.95)
+    reqs = [SolveRequest(T1, M1, (1.0, 1.0, 1.0), "gradient", {"it_max": 200}, request_id="g"),
+            SolveRequest(T2, M2, (1.0, 1.0, 1.2), "matching", {"it_max": 60, "epsilon": 1e-3}, request_id="m")]
+    with SolveServer(max_slots=2) as srv:
+        out = {r.request_id: r for r in map(decode_response, srv.handle([encode_request(r) for r in reqs]))}
+    assert set(out) == {"g", "m"} and out["g"].status == 0 and out["m"].status == 0
+    fm, im, _ = gpu_lib.matching_host(T2, (1.0, 1.0, 1.2), it_max=60, epsilon=1e-3)
+    res = out["m"]
+    np.testing.assert_array_equal(res.field, fm)
+    assert (res.reason, res.iterations) == (im["reason"], im["iterations"])
+    assert res.final_max_error == im["final_max_error"]
+    assert res.evaluations == im["cost_passes"] + im["gradient_passes"]

@@ -54,3 +54,18 @@ def test_malformed_messages():
     with pytest.raises(ValueError):
         encode_request(SolveRequest(target=np.ones((3, 3, 3), np.float32), mask=np.ones((3, 3, 3), bool),
                                     params={"learning_rate": 1.0}))
+
+
+def test_matching_request_and_unknown_algorithm():
+    r = _request(False)
+    r.algorithm = "matching"
+    r.params = {"epsilon": 1e-3, "delta": 1e-3, "zeta": 10.0, "it_max": 50}
+    d = decode_request(encode_request(r))
+    assert d.algorithm == "matching" and d.params == r.params
+    np.testing.assert_array_equal(d.target, r.target)
+    r.algorithm = "simulated-annealing"
+    with pytest.raises(ValueError):
+        encode_request(r)
+    bad = encode_request(_request(False)).replace(b'"algorithm": "greedy"', b'"algorithm": "nope"')
+    with pytest.raises(ValueError):
+        decode_request(bad)
