@@ -65,9 +65,9 @@ def parse():
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
                     help="volumes of the batched-vs-plain comparison (0 to skip it)")
-    ap.add_argument("--serve", type=int, default=0,
+    ap.add_argument("--serve", type=int, default=100,
                     help="requests of the serving stream (0 to skip it)")
-    ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--qps", type=float, default=80.0)
     ap.add_argument("--graphs", type=int, default=0, help="engine replays chunks from hipGraphs")
     ap.add_argument("--slots", type=int, default=0, help="engine slots (instances solved at once; 0: 256)")
     ap.add_argument("--serve-dump", default="", help="write per-request records of the serving run (JSON)")
@@ -154,7 +154,13 @@ def batch_bench(count, it_max, graphs):
                                       F.data_ptr(), stream=sh, it_max=it_max)
         return info
 
-    # warm-up of both paths on the first volume
+    # warm-up of both paths on the first volume; the solver's device arena is
+    # sized once for the largest volume (a 1-iteration solve), so the timed
+    # plain solves do not pay its growth
+    big = max(range(len(vols)), key=lambda i: vols[i][0].numel())
+    T, M, F, sp = vols[big]
+    native.solve_device("gradient", T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
+                        F.data_ptr(), stream=sh, it_max=1)
     vols[0][2].zero_()
     plain(0)
     with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
@@ -255,6 +261,16 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
     dT = torch.empty(nmax, dtype=torch.float32, device=dev)
     dM = torch.empty(nmax, dtype=torch.uint8, device=dev)
     dF = torch.empty(3 * nmax, dtype=torch.float32, device=dev)
+    # size the solver's device arena for the largest request before the
+    # stream starts (a server allocates up front; a first-time growth after a
+    # large free can take hundreds of milliseconds)
+    side_max = max(r["side"] for r in reqs)
+    dT[:nmax].fill_(1.0)
+    dM[:nmax].zero_()
+    dF[:3 * nmax].zero_()
+    native.solve_device("gradient", side_max, side_max, side_max, (1.0, 1.0, 1.0), dT.data_ptr(), dM.data_ptr(),
+                        dF.data_ptr(), stream=sh, it_max=1)
+    torch.cuda.synchronize()
     lat_plain, vi_plain, info_plain, busy = [], 0, [], 0.0
     start = time.perf_counter()
     for i, r in enumerate(reqs):
@@ -433,7 +449,8 @@ def main():
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
     batch = batch_bench(args.batch, args.it_max, args.graphs) if args.batch > 0 else None
-    serving = (serve_bench(args.serve, args.qps, args.it_max, args.graphs, slots=args.slots)
+    serving = (serve_bench(args.serve, args.qps, args.it_max, args.graphs, slots=args.slots,
+                           dump=args.serve_dump)
                if args.serve > 0 else None)
 
     threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
