@@ -1,0 +1,43 @@
+"""Seeded random cases against the oracle: shapes 3..41 per axis, anisotropic
+spacing, random masks and targets, and random solver parameters (alpha,
+beta, eta, strict, it_max), gradient and greedy.  Bar: the stop reason of
+the FP64 or the FP32 oracle (they may disagree at rounding level), a final
+masked max error within 1e-4 of the FP64 oracle's, a finite field."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(seed=7, count=30):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        shape = tuple(int(v) for v in rng.integers(3, 42, size=3))
+        alg = ["gradient", "greedy"][int(rng.integers(0, 2))]
+        sp = tuple(float(v) for v in rng.uniform(0.6, 1.6, size=3))
+        T = (1.0 + rng.uniform(0.02, 0.12) * rng.standard_normal(shape)).astype(np.float32)
+        M = rng.random(shape) < rng.uniform(0.3, 1.0)
+        params = dict(it_max=int(rng.integers(20, 200)), alpha=float(rng.uniform(1.05, 1.5)),
+                      beta=float(rng.uniform(0.3, 0.7)), eta=float(rng.uniform(0.05, 0.3)),
+                      strict=bool(rng.random() < 0.3))
+        out.append((alg, T, M, sp, params))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("k", range(len(CASES)))
+def test_random_case(gpu_lib, oracle32, oracle64, k):
+    alg, T, M, sp, params = CASES[k]
+    print("random case", k, alg, T.shape, params, flush=True)
+    fo, io = oracle64.solve(alg, T.astype(np.float64), M, sp, **params)
+    _, i32 = oracle32.solve(alg, T, M, sp, **params)
+    fg, ig, _ = gpu_lib.solve_host(alg, T, M, sp, **params)
+    assert ig["reason"] in (io["reason"], i32["reason"]), (ig["reason"], io["reason"], i32["reason"])
+    _, mo, _ = oracle32.compute_error(T, oracle32.jacobian(fo.astype(np.float32), sp), M, 0.2)
+    _, mg, _ = oracle32.compute_error(T, oracle32.jacobian(fg, sp), M, 0.2)
+    assert abs(mg - mo) <= 1e-4
+    assert np.isfinite(fg).all()
