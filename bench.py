@@ -54,8 +54,10 @@ MEASURED_TRAFFIC_128 = 86.76e6
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # a 128^3 gradient solve takes ~1.6 ms: 20 of them keep the timed region
+    # well above the clock's and the host's jitter
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--it-max", type=int, default=2000)
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
