@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy", "pyramid"), default="solve")
     ap.add_argument("--algorithm", choices=("gradient", "greedy", "matching"), default="gradient",
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
@@ -359,6 +359,30 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
     }
 
 
+def pyramid_bench(it_max, reps=3):
+    """displacement() on the 128^3 workload (numpy in and out) with 1 and with
+    3 pyramid levels: the multi-resolution driver keeps the pyramid and the
+    field on the device (one upload, one download per call)."""
+    import numpy as np
+    from disptools_amd import native
+    from disptools_amd.displacements import displacement
+    from disptools_amd.workloads import volume_128
+
+    native.set_device(0)
+    T, M, sp = volume_128()
+    out = {"workload": "displacement() of the 128^3 gradient workload, numpy in/out, it_max=%d per level"
+                       % it_max}
+    for levels in (1, 3):
+        displacement(T, mask=M, spacing=sp, levels=levels, it_max=it_max, gpu_id=0)   # warm-up
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f = displacement(T, mask=M, spacing=sp, levels=levels, it_max=it_max, gpu_id=0)
+            walls.append(time.perf_counter() - t0)
+        out["levels_%d" % levels] = {"wall_s": float(np.median(walls)), "finite": bool(np.isfinite(f).all())}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -367,6 +391,9 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         copy_calibration()
+        return
+    if args.mode == "pyramid":
+        print(json.dumps({"mode": "pyramid", **pyramid_bench(args.it_max)}))
         return
     if args.mode in ("batch", "serve"):
         from disptools_amd import native
