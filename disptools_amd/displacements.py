@@ -192,8 +192,9 @@ def _pyramid_displacement(J, M, guess, spacing, levels, it_max, parameters):
         sp_levels.append(sp_l)
     field, f_sp = None, tuple(spacing)
     if guess is not None:
-        g = np.ascontiguousarray(np.moveaxis(np.asarray(guess, np.float32), -1, 0))
-        field = torch.from_numpy(g).to(dev)
+        # [z, y, x, 3] -> [3, z, y, x] on the device
+        g = torch.from_numpy(np.ascontiguousarray(guess, np.float32)).to(dev)
+        field = g.permute(3, 0, 1, 2).contiguous()
     params = {k: v for k, v in parameters.items() if k not in ("algorithm", "gpu_id")}
     stream = torch.cuda.current_stream(dev).cuda_stream
     # it_max[k] belongs to the k-th level from the coarsest (reference :476 zips
@@ -215,7 +216,8 @@ def _pyramid_displacement(J, M, guess, spacing, levels, it_max, parameters):
             native.solve_device(parameters["algorithm"], mx, my, mz, sp_levels[level], T.data_ptr(),
                                 Mb.data_ptr(), field.data_ptr(), stream=stream, it_max=it_max[k], **params)
         f_sp = sp_levels[level]
-    return np.ascontiguousarray(np.moveaxis(field.cpu().numpy(), 0, -1))
+    # [3, z, y, x] -> [z, y, x, 3] on the device, then one download
+    return field.permute(1, 2, 3, 0).contiguous().cpu().numpy()
 
 
 def displacement(
