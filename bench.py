@@ -131,10 +131,11 @@ def _iterations(info):
     return info["iterations"] - (1 if info["reason"] == 6 else 0)
 
 
-def batch_bench(count, it_max, graphs):
+def batch_bench(count, it_max, graphs, algorithm="gradient"):
     """64 volumes of uneven sizes (each axis U{48..160}, spacing 1): solved one
-    dt_solve_device call after the other (the plain one-volume-per-launch
-    solve) and all at once by the batch engine, inputs resident in HBM."""
+    dt_solve_device (matching: dt_matching_device) call after the other (the
+    plain one-volume-per-launch solve) and all at once by the batch engine,
+    inputs resident in HBM."""
     import numpy as np
     import torch
     from disptools_amd import native
@@ -149,26 +150,31 @@ def batch_bench(count, it_max, graphs):
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
 
-    def plain(i):
+    def plain(i, its=it_max):
         T, M, F, sp = vols[i]
         nz, ny, nx = T.shape
-        info, _ = native.solve_device("gradient", nx, ny, nz, sp, T.data_ptr(), M.data_ptr(),
-                                      F.data_ptr(), stream=sh, it_max=it_max)
+        if algorithm == "matching":
+            info, _ = native.matching_device(nx, ny, nz, sp, T.data_ptr(), F.data_ptr(), stream=sh, it_max=its)
+            return info
+        info, _ = native.solve_device(algorithm, nx, ny, nz, sp, T.data_ptr(), M.data_ptr(),
+                                      F.data_ptr(), stream=sh, it_max=its)
         return info
+
+    # matching counts accepted iterations
+    count_its = (lambda i: i["iterations"]) if algorithm == "matching" else _iterations
 
     # warm-up of both paths on the first volume; the solver's device arena is
     # sized once for the largest volume (a 1-iteration solve), so the timed
     # plain solves do not pay its growth
     big = max(range(len(vols)), key=lambda i: vols[i][0].numel())
-    T, M, F, sp = vols[big]
-    native.solve_device("gradient", T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
-                        F.data_ptr(), stream=sh, it_max=1)
+    plain(big, 1)
+    vols[big][2].zero_()
     vols[0][2].zero_()
     plain(0)
     with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
         vols[0][2].zero_()
         T, M, F, sp = vols[0]
-        eng.submit("gradient", T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
+        eng.submit(algorithm, T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
                    F.data_ptr(), it_max=it_max)
         eng.drain()
         eng.poll()
@@ -182,6 +188,7 @@ def batch_bench(count, it_max, graphs):
     torch.cuda.synchronize()
     plain_wall = time.perf_counter() - t0
     plain_max = [i["final_max_error"] for i in plain_infos]
+    plain_fields = [v[2].clone() for v in vols]
 
     for v in vols:
         v[2].zero_()
@@ -191,7 +198,7 @@ def batch_bench(count, it_max, graphs):
         tickets = {}
         for i, (T, M, F, sp) in enumerate(vols):
             nz, ny, nx = T.shape
-            t = eng.submit("gradient", nx, ny, nz, sp, T.data_ptr(), M.data_ptr(), F.data_ptr(),
+            t = eng.submit(algorithm, nx, ny, nz, sp, T.data_ptr(), M.data_ptr(), F.data_ptr(),
                            it_max=it_max)
             if t is None:
                 raise RuntimeError("the engine's pool cannot hold the batch")
@@ -204,23 +211,25 @@ def batch_bench(count, it_max, graphs):
     batch_infos = [res[i][1] for i in range(len(vols))]
 
     voxels = [int(np.prod(v[0].shape)) for v in vols]
-    plain_vi = sum(n * _iterations(i) for n, i in zip(voxels, plain_infos))
-    batch_vi = sum(n * _iterations(i) for n, i in zip(voxels, batch_infos))
+    plain_vi = sum(n * count_its(i) for n, i in zip(voxels, plain_infos))
+    batch_vi = sum(n * count_its(i) for n, i in zip(voxels, batch_infos))
+    identical = sum(1 for a, v in zip(plain_fields, vols) if torch.equal(a, v[2]))
     diffs = [abs(b["final_max_error"] - p) for b, p in zip(batch_infos, plain_max)]
     same_reason = sum(1 for b, p in zip(batch_infos, plain_infos) if b["reason"] == p["reason"])
     return {
-        "workload": "%d gradient solves, each axis U{48..160}, spacing 1, smoothed log-normal target, "
-                    "ellipsoid mask, zero init, it_max=%d" % (len(vols), it_max),
+        "workload": "%d %s solves, each axis U{48..160}, spacing 1, smoothed log-normal target, "
+                    "ellipsoid mask, zero init, it_max=%d" % (len(vols), algorithm, it_max),
         "voxels": sum(voxels),
         "plain": {"wall_s": plain_wall, "voxel_iters_per_s": plain_vi / plain_wall,
-                  "iterations": sum(_iterations(i) for i in plain_infos)},
+                  "iterations": sum(count_its(i) for i in plain_infos)},
         "batched": {"wall_s": batch_wall, "voxel_iters_per_s": batch_vi / batch_wall,
-                    "iterations": sum(_iterations(i) for i in batch_infos),
+                    "iterations": sum(count_its(i) for i in batch_infos),
                     "chunks": stats["chunks"], "pass_launches": stats["passes"],
                     "graph_builds": stats["graph_builds"], "pool_peak_bytes": stats["pool_peak"],
                     "failed": sum(1 for i in range(len(vols)) if res[i][0] != 0)},
         "speedup": (batch_vi / batch_wall) / (plain_vi / plain_wall),
         "same_reason": "%d/%d" % (same_reason, len(vols)),
+        "identical_fields": "%d/%d" % (identical, len(vols)),
         "max_abs_diff_final_max_error": max(diffs),
     }
 
@@ -400,7 +409,7 @@ def main():
         if not native.set_device(0):
             raise RuntimeError("no HIP device")
         if args.mode == "batch":
-            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs)))
+            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm)))
         else:
             print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
                                          slots=args.slots, dump=args.serve_dump)))

@@ -34,6 +34,7 @@
 
 #include "disptools_hip.h"
 #include "dt_host.hpp"
+#include "dt_match.hpp"
 
 namespace dt {
 
@@ -46,7 +47,7 @@ enum SlotState { SLOT_FREE = 0, SLOT_ACTIVE = 1, SLOT_RETIRING = 2 };
 struct Slot {
     int state = SLOT_FREE;
     long long ticket = -1;
-    int lane = 0;                 // DT_GRADIENT or DT_GREEDY
+    int lane = 0;                 // DT_GRADIENT, DT_GREEDY or DT_MATCHING
     Plan P;
     dt_params params{};
     int host_io = 0;
@@ -55,6 +56,9 @@ struct Slot {
     long long launched = 0;       // passes launched over this instance
     int failed = 0;
     Ctrl final_ctrl{};
+    MCtrl final_mctrl{};          // matching instances
+    float *dev_field = nullptr;   // matching: the field on the device (positions -> field at the end)
+    float spacing[3] = {1.0f, 1.0f, 1.0f};
     Clock::time_point t_submit;
     hipEvent_t ev_ready = nullptr;   // inputs uploaded (host_io)
     hipEvent_t ev_stop = nullptr;    // the lane's stream reached the retirement point
@@ -143,7 +147,7 @@ struct Pool {
 
 constexpr size_t POOL_GRAIN = (size_t)1 << 16;
 constexpr int MAX_SLOTS = 4096;
-constexpr int NLANES = 2;
+constexpr int NLANES = 3;   // DT_GRADIENT, DT_GREEDY, DT_MATCHING
 
 }  // namespace
 
@@ -170,12 +174,17 @@ struct dt_engine {
     Ctrl *d_ctrl = nullptr;
     unsigned int *d_counter = nullptr;
     Inst *h_insts = nullptr;     // pinned mirror, one per slot
+    MInst *d_minsts = nullptr;   // matching instances, slot-indexed
+    MCtrl *d_mctrl = nullptr;
+    MInst *h_minsts = nullptr;   // pinned mirror
+    MCtrl *h_mctrl0 = nullptr;   // pinned initial control blocks
     Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks
     int *h_zero = nullptr;       // pinned 0 (deactivates a stuck instance)
     // host-coherent, per slot: completion words, then final control blocks
     char *coherent = nullptr;
     char *coherent_dev = nullptr;
     size_t final_off = 0;
+    size_t mfinal_off = 0;       // published matching control blocks
     bool status_words = true;    // the kernels publish them (DISPTOOLS_STATUS_WORD)
     std::chrono::steady_clock::time_point last_check{};   // of check_lanes
 
@@ -192,6 +201,14 @@ namespace {
 
 volatile unsigned long long *status_of(dt_engine *e, int s) { return (volatile unsigned long long *)e->coherent + s; }
 Ctrl *final_of(dt_engine *e, int s) { return (Ctrl *)(e->coherent + e->final_off) + s; }
+MCtrl *mfinal_of(dt_engine *e, int s) { return (MCtrl *)(e->coherent + e->mfinal_off) + s; }
+
+// Tiles (blocks per pass) of the instance in slot s.
+int slot_tiles(const dt_engine *e, int s)
+{
+    const Slot &sl = e->slots[s];
+    return sl.lane == DT_MATCHING ? e->h_minsts[s].nblocks : sl.P.ntiles;
+}
 
 int engine_tz(const dt_engine *e, int nz)
 {
@@ -217,6 +234,13 @@ size_t footprint(const dt_engine *e, const Plan &P, int host_io)
 {
     (void)e;
     const size_t b = (host_io ? io_bytes(P.n) : 0) + scratch_bytes(P, 0, false);
+    return align_up(b, POOL_GRAIN);
+}
+
+size_t match_footprint(int nx, int ny, int nz, int host_io)
+{
+    const long long n = (long long)nx * ny * nz;
+    const size_t b = (host_io ? io_bytes(n) : 0) + match_scratch_bytes(nx, ny, nz);
     return align_up(b, POOL_GRAIN);
 }
 
@@ -253,8 +277,9 @@ int rebuild_tables(dt_engine *e, Lane &L, int stage)
     for (int s = 0; s < (int)e->slots.size(); ++s) {
         const Slot &sl = e->slots[s];
         if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
-        seg.push_back({s, sl.P.ntiles});
-        count += (size_t)sl.P.ntiles;
+        const int nt = slot_tiles(e, s);
+        seg.push_back({s, nt});
+        count += (size_t)nt;
     }
     if (ensure_tables(L, std::max<size_t>(count, 1))) return -1;
     std::vector<Tile> t;
@@ -279,8 +304,10 @@ int launch_passes(dt_engine *e, const Lane &L, hipStream_t s)
     for (int i = 0; i < e->chunk; ++i) {
         if (L.kind == DT_GRADIENT) {
             DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, L.d_tiles, L.ntiles, s));
-        } else {
+        } else if (L.kind == DT_GREEDY) {
             DT_HIP(launch_greedy_pass(e->precision, e->greedy_variant, e->d_insts, L.d_tiles, L.ntiles, s));
+        } else {
+            DT_HIP(launch_match_batch(e->d_minsts, L.d_tiles, L.ntiles, s));
         }
     }
     return 0;
@@ -348,6 +375,22 @@ int retire(dt_engine *e, int s, const Ctrl &fin, int failed)
     L.n_active--;
     e->n_retiring++;
     L.dirty = true;
+    if (sl.lane == DT_MATCHING) {
+        // accepted node positions -> field on the device, behind the passes
+        const MInst &h = e->h_minsts[s];
+        DT_HIP(launch_match_out(h, e->d_minsts + s, sl.dev_field, sl.spacing[0], sl.spacing[1], sl.spacing[2],
+                                L.st));
+        const size_t mbytes = 3 * (size_t)h.n * sizeof(float);
+        if (sl.host_io) {
+            DT_HIP(hipEventRecord(sl.ev_stop, L.st));
+            DT_HIP(hipStreamWaitEvent(e->io, sl.ev_stop, 0));
+            DT_HIP(hipMemcpyAsync(sl.out_field, sl.dev_field, mbytes, hipMemcpyDeviceToHost, e->io));
+            DT_HIP(hipEventRecord(sl.ev_done, e->io));
+        } else {
+            DT_HIP(hipEventRecord(sl.ev_done, L.st));
+        }
+        return 0;
+    }
     const int rb = (fin.result_buf >= 0 && fin.result_buf < NBUF) ? fin.result_buf : 0;
     const size_t bytes = 3 * (size_t)sl.P.n * sizeof(float);
     float *src = e->h_insts[s].buf[rb];
@@ -370,17 +413,29 @@ void finish(dt_engine *e, int s)
     dt_engine_result r{};
     r.ticket = sl.ticket;
     r.status = sl.failed ? -1 : 0;
-    const Ctrl &f = sl.final_ctrl;
-    r.info.iterations = f.it;
-    r.info.evaluations = f.evaluations;
-    r.info.rejections = f.rejections;
+    if (sl.lane == DT_MATCHING) {
+        // matching: evaluations = cost + gradient passes, final_error = final cost
+        const MCtrl &f = sl.final_mctrl;
+        r.info.iterations = f.it;
+        r.info.evaluations = f.cost_passes + f.grad_passes;
+        r.info.rejections = 0;
+        r.info.reason = sl.failed ? DT_RUNNING : f.reason;
+        r.info.final_error = f.H;
+        r.info.final_max_error = f.m;
+        r.info.field_io = 0;
+    } else {
+        const Ctrl &f = sl.final_ctrl;
+        r.info.iterations = f.it;
+        r.info.evaluations = f.evaluations;
+        r.info.rejections = f.rejections;
+        r.info.reason = sl.failed ? DT_RUNNING : f.reason;
+        r.info.final_error = f.E_result;
+        r.info.final_max_error = f.m_result;
+        r.info.field_io = f.field_io;
+    }
     r.info.launches = sl.launched;
-    r.info.reason = sl.failed ? DT_RUNNING : f.reason;
-    r.info.final_error = f.E_result;
-    r.info.final_max_error = f.m_result;
     r.info.seconds = std::chrono::duration<double>(Clock::now() - sl.t_submit).count();
     r.info.trace_len = 0;
-    r.info.field_io = f.field_io;
     e->results.push_back(r);
     e->pool.release(sl.pool_off, sl.pool_bytes);
     sl.state = SLOT_FREE;
@@ -412,8 +467,9 @@ int retire_done(dt_engine *e)
         if (sl.state != SLOT_ACTIVE || !(*status_of(e, s) & 1ull)) continue;
         // the device stores the control block before the word
         std::atomic_thread_fence(std::memory_order_acquire);
-        Ctrl f;
-        std::memcpy(&f, final_of(e, s), sizeof(Ctrl));
+        Ctrl f{};
+        if (sl.lane == DT_MATCHING) std::memcpy(&sl.final_mctrl, mfinal_of(e, s), sizeof(MCtrl));
+        else std::memcpy(&f, final_of(e, s), sizeof(Ctrl));
         if (retire(e, s, f, 0)) return -1;
         ++k;
     }
@@ -439,6 +495,14 @@ int check_stuck(dt_engine *e, Lane &L, const Chunk &c)
     for (const Need &n : c.needs) {
         Slot &sl = e->slots[n.slot];
         if (sl.state != SLOT_ACTIVE || sl.ticket != n.ticket) continue;
+        if (sl.lane == DT_MATCHING) {
+            if (sl.launched <= match_max_passes(sl.params.it_max)) continue;
+            DT_HIP(hipMemcpyAsync(&e->d_mctrl[n.slot].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
+            DT_HIP(hipStreamSynchronize(L.st));
+            DT_HIP(hipMemcpy(&sl.final_mctrl, &e->d_mctrl[n.slot], sizeof(MCtrl), hipMemcpyDeviceToHost));
+            if (retire(e, n.slot, Ctrl{}, sl.final_mctrl.phase != MPH_DONE)) return -1;
+            continue;
+        }
         if (sl.launched <= max_passes(sl.params)) continue;
         DT_HIP(hipMemcpyAsync(&e->d_ctrl[n.slot].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
         DT_HIP(hipStreamSynchronize(L.st));
@@ -481,6 +545,15 @@ int check_lanes(dt_engine *e)
             for (const Need &n : c.needs) {
                 const Slot &sl = e->slots[n.slot];
                 if (sl.state != SLOT_ACTIVE || sl.ticket != n.ticket) continue;
+                if (sl.lane == DT_MATCHING) {
+                    MCtrl m;
+                    DT_HIP(hipMemcpy(&m, &e->d_mctrl[n.slot], sizeof(MCtrl), hipMemcpyDeviceToHost));
+                    const bool mdone = m.phase == MPH_DONE;
+                    if (mdone) std::memcpy(mfinal_of(e, n.slot), &m, sizeof(MCtrl));
+                    *status_of(e, n.slot) =
+                        ((unsigned long long)(m.cost_passes + m.grad_passes) << 1) | (mdone ? 1ull : 0ull);
+                    continue;
+                }
                 Ctrl f;
                 DT_HIP(hipMemcpy(&f, &e->d_ctrl[n.slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
                 const bool done = f.phase == PH_DONE;
@@ -572,6 +645,10 @@ void destroy(dt_engine *e)
     if (e->h_insts) (void)hipHostFree(e->h_insts);
     if (e->h_ctrl0) (void)hipHostFree(e->h_ctrl0);
     if (e->h_zero) (void)hipHostFree(e->h_zero);
+    if (e->h_minsts) (void)hipHostFree(e->h_minsts);
+    if (e->h_mctrl0) (void)hipHostFree(e->h_mctrl0);
+    if (e->d_minsts) (void)hipFree(e->d_minsts);
+    if (e->d_mctrl) (void)hipFree(e->d_mctrl);
     if (e->coherent) (void)hipHostFree(e->coherent);
     if (e->d_insts) (void)hipFree(e->d_insts);
     if (e->d_ctrl) (void)hipFree(e->d_ctrl);
@@ -637,10 +714,17 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
     DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
     *e->h_zero = 0;
+    DT_HIP(hipMalloc(&e->d_minsts, (size_t)S * sizeof(MInst)));
+    DT_HIP(hipMalloc(&e->d_mctrl, (size_t)S * sizeof(MCtrl)));
+    DT_HIP(hipMemset(e->d_mctrl, 0, (size_t)S * sizeof(MCtrl)));
+    DT_HIP(hipHostMalloc((void **)&e->h_minsts, (size_t)S * sizeof(MInst), hipHostMallocDefault));
+    DT_HIP(hipHostMalloc((void **)&e->h_mctrl0, (size_t)S * sizeof(MCtrl), hipHostMallocDefault));
     e->final_off = align_up((size_t)S * sizeof(unsigned long long), 256);
-    DT_HIP(host_words_alloc((void **)&e->coherent, e->final_off + (size_t)S * sizeof(Ctrl)));
+    e->mfinal_off = align_up(e->final_off + (size_t)S * sizeof(Ctrl), 256);
+    const size_t coherent_bytes = e->mfinal_off + (size_t)S * sizeof(MCtrl);
+    DT_HIP(host_words_alloc((void **)&e->coherent, coherent_bytes));
     DT_HIP(hipHostGetDevicePointer((void **)&e->coherent_dev, e->coherent, 0));
-    std::memset(e->coherent, 0, e->final_off + (size_t)S * sizeof(Ctrl));
+    std::memset(e->coherent, 0, coherent_bytes);
     e->slots.resize(S);
     for (auto &sl : e->slots) {
         DT_HIP(hipEventCreateWithFlags(&sl.ev_ready, hipEventDisableTiming));
@@ -651,10 +735,98 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     return 0;
 }
 
+// A matching solve (src/VolumeMatching3D.c:689-796): epsilon, delta, zeta
+// and it_max of `params`; the mask is not read, as in the reference.
+long long submit_matching(dt_engine *e, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                          const float *target, const dt_params *params, float *field, int host_io)
+{
+    if (!params) DT_FAIL("null params");
+    if (matching_validate(nx, ny, nz, dx, dy, dz, params->epsilon, params->delta, params->zeta)) return -1;
+    if (!target || !field) DT_FAIL("null buffer");
+    int s = -1;
+    for (int i = 0; i < (int)e->slots.size(); ++i)
+        if (e->slots[i].state == SLOT_FREE) { s = i; break; }
+    if (s < 0) return DT_ENGINE_FULL;
+    const size_t bytes = match_footprint((int)nx, (int)ny, (int)nz, host_io);
+    if (bytes > e->pool.size) DT_FAIL("instance larger than the engine's hbm_budget");
+    size_t off = 0;
+    if (!e->pool.alloc(bytes, off)) return DT_ENGINE_FULL;
+
+    Lane &L = e->lanes[DT_MATCHING];
+    Slot &sl = e->slots[s];
+    sl.lane = DT_MATCHING;
+    sl.P = Plan{};
+    sl.params = *params;
+    sl.host_io = host_io;
+    sl.pool_off = off;
+    sl.pool_bytes = bytes;
+    sl.failed = 0;
+    sl.t_submit = Clock::now();
+    const size_t n = nx * ny * nz;
+    Carver cv(e->pool_mem + off);
+    const float *d_target;
+    float *d_field;
+    int rc = 0;
+    if (host_io) {
+        d_field = cv.take<float>(3 * n);
+        float *t = cv.take<float>(n);
+        if (hipMemcpyAsync(d_field, field, 3 * n * sizeof(float), hipMemcpyHostToDevice, e->io) != hipSuccess ||
+            hipMemcpyAsync(t, target, n * sizeof(float), hipMemcpyHostToDevice, e->io) != hipSuccess ||
+            hipEventRecord(sl.ev_ready, e->io) != hipSuccess ||
+            hipStreamWaitEvent(L.st, sl.ev_ready, 0) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot stage the instance's inputs on the device");
+            rc = -1;
+        }
+        d_target = t;
+    } else {
+        d_field = field;
+        d_target = target;
+    }
+    sl.out_field = field;
+    sl.dev_field = d_field;
+    sl.spacing[0] = dx;
+    sl.spacing[1] = dy;
+    sl.spacing[2] = dz;
+    if (!rc) {
+        *status_of(e, s) = 0;
+        MInst &h = e->h_minsts[s];
+        h = match_make_inst((int)nx, (int)ny, (int)nz, d_target, params->epsilon, params->delta, params->zeta,
+                            params->it_max, cv, &e->d_mctrl[s], &e->d_counter[s]);
+        if (e->status_words) {
+            h.status = (unsigned long long *)e->coherent_dev + s;
+            h.final_ctrl = (MCtrl *)(e->coherent_dev + e->mfinal_off) + s;
+        }
+        e->h_mctrl0[s] = match_initial_ctrl();
+        // field -> node positions, then the passes (same stream)
+        if (hipMemcpyAsync(&e->d_minsts[s], &h, sizeof(MInst), hipMemcpyHostToDevice, L.st) != hipSuccess ||
+            hipMemcpyAsync(&e->d_mctrl[s], &e->h_mctrl0[s], sizeof(MCtrl), hipMemcpyHostToDevice, L.st) !=
+                hipSuccess ||
+            hipMemsetAsync(&e->d_counter[s], 0, sizeof(unsigned int), L.st) != hipSuccess ||
+            launch_match_in(h, d_field, dx, dy, dz, L.st) != hipSuccess) {
+            set_error(__LINE__, __FILE__, __func__, "cannot upload the instance record");
+            rc = -1;
+        }
+    }
+    if (rc) {
+        e->pool.release(off, bytes);
+        return -1;
+    }
+    sl.state = SLOT_ACTIVE;
+    sl.ticket = e->next_ticket++;
+    sl.launched = 0;
+    e->n_active++;
+    L.n_active++;
+    L.dirty = true;
+    e->stats.admitted++;
+    return sl.ticket;
+}
+
 long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
                  const float *target, const unsigned char *mask, const dt_params *params, float *field,
                  int host_io)
 {
+    if (algorithm == DT_MATCHING)
+        return submit_matching(e, nx, ny, nz, dx, dy, dz, target, params, field, host_io);
     if (validate(algorithm, nx, ny, nz, dx, dy, dz, params)) return -1;
     if (!target || !mask || !field) DT_FAIL("null buffer");
     int s = -1;
@@ -770,8 +942,11 @@ extern "C" void dt_engine_destroy(dt_engine *e) { destroy(e); }
 extern "C" size_t dt_engine_footprint(const dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz,
                                       int host_io)
 {
-    if (!e || nx < 3 || ny < 3 || nz < 3) return 0;
+    if (!e) return 0;
     (void)hipSetDevice(e->device);
+    if (algorithm == DT_MATCHING)
+        return nx < 2 || ny < 2 || nz < 2 ? 0 : match_footprint((int)nx, (int)ny, (int)nz, host_io);
+    if (nx < 3 || ny < 3 || nz < 3) return 0;
     return footprint(e, engine_plan(e, algorithm, (int)nx, (int)ny, (int)nz), host_io);
 }
 

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "dt_host.hpp"
+#include "dt_match.hpp"
 
 namespace dt {
 namespace {
@@ -50,39 +51,7 @@ constexpr int NPX = MX + 1, NPP = NPX * (MY + 1);  // nodes of a cell tile's pla
 constexpr int NEXTRA = NPP - MNT;                  // node points loaded by a second slot
 static_assert(NEXTRA <= MNT, "one extra point per thread at most");
 
-enum { MPH_INIT = 0, MPH_GRAD = 1, MPH_COST = 2, MPH_DONE = 3 };
 enum { MM_FIRST = 0, MM_GROW = 1, MM_SHRINK = 2 };
-
-struct MCtrl {
-    int phase, active, mode;
-    int xb, tb, pb;            // accepted triple, trial triple, previous trial (growth)
-    float alpha, m;            // step; max voxel error of the last cost pass
-    double H, Hmark;           // cost of the accepted point; line-search reference
-    long long it, cost_passes, grad_passes, trace_len;
-    int reason;
-};
-
-struct PassGeom {
-    int ntx, nty, ntz, tz;
-};
-
-struct MInst {
-    int nx, ny, nz;
-    PassGeom cg, gg;           // tilings of the cost and gradient passes
-    float delta, zeta, epsilon;
-    long long it_max;
-    long long n;
-    float *tri[3];             // triple t: F, G, E at tri[t] + {0, n, 2n}
-    float *w[3];               // cell weights of triple t, node-indexed
-    float *D;                  // gradient, F, G, E at D + {0, n, 2n}
-    const float *V;            // target, node-indexed
-    double *partials;          // 2 per block
-    unsigned *counter;
-    MCtrl *ctrl;
-    double *trace;
-    long long trace_cap;
-    unsigned long long *status;   // host-coherent completion word: (passes << 1) | done
-};
 
 // Instance record, control block and arrival counter of a matching solve,
 // staged by one copy.
@@ -502,23 +471,17 @@ __device__ void match_controller(const MInst &I, MCtrl *C, double nH, float nm)
     }
 }
 
-__global__ void __launch_bounds__(MNT)
-k_match_pass(const MInst *__restrict__ pinst)
+// One block of a pass over tile t of instance I, whose passes launch
+// `nblocks` blocks: the tile's part of the cost or gradient pass, its partial
+// sums, and in the last block to arrive the fold of all partials (tile order)
+// and the step control.
+__device__ __forceinline__ void match_block(const MInst &I, int t, int nblocks, PassSmem &S, double *red, int &last)
 {
-    __shared__ PassSmem S;
-    __shared__ double red[2 * (MNT / 64)];
-    __shared__ int last;
-    const MInst &I = *pinst;
     MCtrl *C = I.ctrl;
     if (!C->active) return;
     const MCtrl ctl = *C;
     const bool grad = ctl.phase == MPH_GRAD;
     const PassGeom G = grad ? I.gg : I.cg;
-    // blocks are dispatched to the 8 XCDs round robin: give each XCD a
-    // contiguous range of tiles, so that neighbouring tiles share an L2 and
-    // their halo loads hit it (the grid is a multiple of 8)
-    const int per_xcd = (int)gridDim.x / 8;
-    const int t = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     double Hs = 0.0;
     float mx = 0.0f;
     if (t < G.ntx * G.nty * G.ntz) {
@@ -543,13 +506,13 @@ k_match_pass(const MInst *__restrict__ pinst)
         __hip_atomic_store(&I.partials[2 * t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&I.partials[2 * t + 1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const unsigned prev = __hip_atomic_fetch_add(I.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = prev + 1 == (unsigned)gridDim.x;
+        last = prev + 1 == (unsigned)nblocks;
     }
     __syncthreads();
     if (!last) return;
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
     double a = 0.0, b = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += MNT) {
+    for (int k = threadIdx.x; k < nblocks; k += MNT) {
         a += __hip_atomic_load(&I.partials[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         b = fmax(b, __hip_atomic_load(&I.partials[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
@@ -565,12 +528,49 @@ k_match_pass(const MInst *__restrict__ pinst)
         double sa = 0.0, sb = 0.0;
         for (int k = 0; k < MNT / 64; ++k) { sa += red[2 * k]; sb = fmax(sb, red[2 * k + 1]); }
         match_controller(I, C, sa, (float)sb);
+        const bool done = C->phase == MPH_DONE;
+        if (done && I.final_ctrl) {
+            // the control block first, ordered before the word
+            const unsigned long long *src = (const unsigned long long *)C;
+            unsigned long long *dst = (unsigned long long *)I.final_ctrl;
+            for (int k = 0; k < (int)(sizeof(MCtrl) / 8); ++k)
+                __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __atomic_thread_fence(__ATOMIC_RELEASE);
+        }
         const unsigned long long passes = (unsigned long long)(C->cost_passes + C->grad_passes);
         if (I.status)
-            __hip_atomic_store(I.status, (passes << 1) | (C->phase == MPH_DONE ? 1ull : 0ull), __ATOMIC_RELAXED,
+            __hip_atomic_store(I.status, (passes << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// A pass of one instance: blocks are dispatched to the 8 XCDs round robin,
+// so each XCD gets a contiguous range of tiles (neighbouring tiles share an
+// L2 and their halo loads hit it; the grid is a multiple of 8).
+__global__ void __launch_bounds__(MNT)
+k_match_pass(const MInst *__restrict__ pinst)
+{
+    __shared__ PassSmem S;
+    __shared__ double red[2 * (MNT / 64)];
+    __shared__ int last;
+    const int per_xcd = (int)gridDim.x / 8;
+    const int t = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    match_block(*pinst, t, (int)gridDim.x, S, red, last);
+}
+
+// A pass over the tiles of several instances (the engine's tile table: one
+// entry per block, the instance's tiles 0 .. nblocks-1 each).
+__global__ void __launch_bounds__(MNT)
+k_match_pass_batch(const MInst *__restrict__ insts, const Tile *__restrict__ tiles)
+{
+    __shared__ PassSmem S;
+    __shared__ double red[2 * (MNT / 64)];
+    __shared__ int last;
+    const Tile tl = tiles[blockIdx.x];
+    if (tl.inst < 0) return;
+    const MInst &I = insts[tl.inst];
+    match_block(I, tl.tile, I.nblocks, S, red, last);
 }
 
 // field -> node positions (volume_matching_3d, :746-757)
@@ -678,6 +678,8 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     h.it_max = it_max < 0 ? 0 : it_max;
     h.n = n;
     const int ntiles = (std::max(h.cg.ntx * h.cg.nty * h.cg.ntz, h.gg.ntx * h.gg.nty * h.gg.ntz) + 7) / 8 * 8;
+    h.nblocks = ntiles;
+    h.final_ctrl = nullptr;
     if (trace_cap < 0 || !trace) trace_cap = 0;
 
     size_t bytes = 0;
@@ -812,8 +814,90 @@ static int matching_device(int nx, int ny, int nz, float dx, float dy, float dz,
     return 0;
 }
 
-static int matching_validate(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float epsilon,
-                             float delta, float zeta)
+size_t match_scratch_bytes(int nx, int ny, int nz)
+{
+    const long long n = (long long)nx * ny * nz;
+    const int resident = resident_match_blocks();
+    const PassGeom cg = pass_geom(nx, ny, nz, MX, MY, resident);
+    const PassGeom gg = pass_geom(nx, ny, nz, MX - 1, MY - 1, resident);
+    const int ntiles = (std::max(cg.ntx * cg.nty * cg.ntz, gg.ntx * gg.nty * gg.ntz) + 7) / 8 * 8;
+    size_t bytes = 0;
+    auto add = [&](size_t b) { bytes = align_up(bytes, 256) + b; };
+    for (int t = 0; t < 3; ++t) add(12 * (size_t)n);
+    for (int t = 0; t < 3; ++t) add(4 * (size_t)n);
+    add(12 * (size_t)n);
+    add(16 * (size_t)ntiles);
+    return align_up(bytes, 256);
+}
+
+MInst match_make_inst(int nx, int ny, int nz, const float *d_target, float epsilon, float delta, float zeta,
+                      long long it_max, Carver &cv, MCtrl *ctrl, unsigned *counter)
+{
+    MInst h{};
+    const long long n = (long long)nx * ny * nz;
+    h.nx = nx; h.ny = ny; h.nz = nz;
+    const int resident = resident_match_blocks();
+    h.cg = pass_geom(nx, ny, nz, MX, MY, resident);
+    h.gg = pass_geom(nx, ny, nz, MX - 1, MY - 1, resident);
+    h.nblocks = (std::max(h.cg.ntx * h.cg.nty * h.cg.ntz, h.gg.ntx * h.gg.nty * h.gg.ntz) + 7) / 8 * 8;
+    h.delta = delta; h.zeta = zeta; h.epsilon = epsilon;
+    h.it_max = it_max < 0 ? 0 : it_max;
+    h.n = n;
+    for (int t = 0; t < 3; ++t) h.tri[t] = cv.take<float>(3 * (size_t)n);
+    for (int t = 0; t < 3; ++t) h.w[t] = cv.take<float>((size_t)n);
+    h.D = cv.take<float>(3 * (size_t)n);
+    h.partials = cv.take<double>(2 * (size_t)h.nblocks);
+    h.V = d_target;
+    h.ctrl = ctrl;
+    h.counter = counter;
+    h.trace = nullptr;
+    h.trace_cap = 0;
+    h.status = nullptr;
+    h.final_ctrl = nullptr;
+    return h;
+}
+
+MCtrl match_initial_ctrl()
+{
+    MCtrl c{};
+    c.phase = MPH_INIT;
+    c.active = 1;
+    c.xb = 0;
+    c.alpha = 1.0f;
+    return c;
+}
+
+long long match_max_passes(long long it_max)
+{
+    // every iteration takes one gradient pass and at most ~160 cost passes
+    // (the step doubles from 1 at most 128 times and halves at most 31 times
+    // before falling under 1e-9)
+    return 256 + 192 * ((it_max < 0 ? 0 : it_max) + 1);
+}
+
+hipError_t launch_match_in(const MInst &h, const float *d_field, float dx, float dy, float dz, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_match_in, dim3(grid_of(h.n)), dim3(256), 0, st, d_field, h.tri[0], h.nx, h.ny, h.nz, dx,
+                       dy, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_out(const MInst &h, const MInst *d_inst, float *d_field, float dx, float dy, float dz,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_match_out, dim3(grid_of(h.n)), dim3(256), 0, st, d_inst, d_field, h.nx, h.ny, h.nz, dx, dy,
+                       dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_batch(const MInst *insts, const Tile *tiles, int nblocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_match_pass_batch, dim3(nblocks), dim3(MNT), 0, st, insts, tiles);
+    return hipGetLastError();
+}
+
+int matching_validate(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, float epsilon, float delta,
+                      float zeta)
 {
     if (nx < 2 || ny < 2 || nz < 2) DT_FAIL("every image dimension must be at least 2 voxels");
     if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) DT_FAIL("image too large");

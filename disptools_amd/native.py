@@ -31,7 +31,10 @@ lib = ctypes.CDLL(LIB_PATH)
 
 GRADIENT = 0
 GREEDY = 1
+MATCHING = 2
 ALGORITHMS = {"gradient": GRADIENT, "greedy": GREEDY}
+# the batch engine also takes matching solves (epsilon, delta, zeta, it_max; no mask)
+ENGINE_ALGORITHMS = {"gradient": GRADIENT, "greedy": GREEDY, "matching": MATCHING}
 
 REASONS = {
     0: "running",
@@ -397,11 +400,11 @@ class Engine:
             pass
 
     def footprint(self, algorithm, nx, ny, nz, host_io=False):
-        return int(lib.dt_engine_footprint(self._h, ALGORITHMS[algorithm], nx, ny, nz, 1 if host_io else 0))
+        return int(lib.dt_engine_footprint(self._h, ENGINE_ALGORITHMS[algorithm], nx, ny, nz, 1 if host_io else 0))
 
     def submit(self, algorithm, nx, ny, nz, spacing, target, mask, field, host_io=False, **params):
         p = make_params(**params)
-        t = lib.dt_engine_submit(self._h, ALGORITHMS[algorithm], nx, ny, nz, *map(float, spacing),
+        t = lib.dt_engine_submit(self._h, ENGINE_ALGORITHMS[algorithm], nx, ny, nz, *map(float, spacing),
                                  target, mask, ctypes.byref(p), field, 1 if host_io else 0)
         if t == ENGINE_FULL:
             return None

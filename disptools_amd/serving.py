@@ -10,11 +10,10 @@ Both travel as bytes:
     raw arrays, little-endian         request: target f4, mask u1, [field f4]
                                       response: field f4
 
-``SolveServer`` admits decoded gradient and greedy requests into a
+``SolveServer`` admits decoded gradient, greedy and matching requests into a
 continuous-batching engine as slots and device memory free up and hands back
-responses as solves finish; matching requests (the mask is carried and, as
-in the reference, unused) are solved on the device one at a time between
-engine steps.
+responses as solves finish (a matching request carries a mask which, as in
+the reference, is unused).
 """
 
 import json
@@ -156,9 +155,10 @@ class SolveServer:
 
     ``submit`` queues a request; ``step`` admits queued requests into free
     slots (the inputs are staged from pinned host buffers), advances the
-    passes and returns the responses of the solves that finished.  Matching
-    requests wait in a queue of their own; ``step`` solves the oldest one on
-    the device (the engine's passes already in flight keep running meanwhile).
+    passes and returns the responses of the solves that finished.  Gradient,
+    greedy and matching solves run in lanes of their own, each lane's passes
+    covering all of its admitted volumes; a matching solve reads epsilon,
+    delta, zeta and it_max and ignores the mask, as in the reference.
     """
 
     def __init__(self, hbm_budget=0, max_slots=0, **engine_kw):
@@ -168,7 +168,6 @@ class SolveServer:
         self._native = native
         self.engine = native.Engine(hbm_budget=hbm_budget, max_slots=max_slots, **engine_kw)
         self._queue: List[Tuple[SolveRequest, tuple]] = []
-        self._matching: List[SolveRequest] = []
         self._live: Dict[int, Tuple[SolveRequest, tuple]] = {}
 
     def close(self):
@@ -181,14 +180,11 @@ class SolveServer:
         self.close()
 
     def pending(self):
-        return len(self._queue) + len(self._matching) + len(self._live)
+        return len(self._queue) + len(self._live)
 
     def submit(self, req: SolveRequest):
         if req.algorithm not in ALGORITHMS:
             raise ValueError("algorithm must be one of %s" % (ALGORITHMS,))
-        if req.algorithm == "matching":
-            self._matching.append(req)
-            return
         torch = self._torch
         t = torch.from_numpy(np.ascontiguousarray(req.target, np.float32)).pin_memory()
         m = torch.from_numpy(np.ascontiguousarray(req.mask, np.uint8)).pin_memory()
@@ -209,33 +205,9 @@ class SolveServer:
                 return
             self._live[ticket] = self._queue.pop(0)
 
-    def _solve_matching(self, req: SolveRequest) -> SolveResponse:
-        import time
-        t0 = time.perf_counter()
-        keys = ("epsilon", "delta", "zeta", "it_max")
-        kw = {k: req.params[k] for k in keys if k in req.params}
-        status, info = 0, None
-        try:
-            f, info, _ = self._native.matching_host(req.target, req.spacing, field=req.field, **kw)
-        except self._native.HipError:
-            status = -1
-            f = (np.zeros((3,) + tuple(req.target.shape), np.float32) if req.field is None
-                 else np.array(req.field, np.float32, copy=True))
-        if info is None:
-            return SolveResponse(request_id=req.request_id, status=status, field=f,
-                                 seconds=time.perf_counter() - t0)
-        return SolveResponse(request_id=req.request_id, status=status, field=f, reason=info["reason"],
-                             reason_text=self._native.REASONS[info["reason"]], iterations=info["iterations"],
-                             evaluations=info["cost_passes"] + info["gradient_passes"], rejections=0,
-                             final_error=info["final_cost"], final_max_error=info["final_max_error"],
-                             seconds=time.perf_counter() - t0)
-
     def step(self, wait=True) -> List[SolveResponse]:
         self._admit()
         out = []
-        if self._matching:
-            out.append(self._solve_matching(self._matching.pop(0)))
-            wait = False
         if self._live:
             self.engine.progress(wait=wait)
         for ticket, status, info in self.engine.poll():

@@ -87,7 +87,7 @@ void regularise(const size_t nx, const size_t ny, const size_t nz, float *J, flo
 /* Additive entry points (no counterpart in the reference)                 */
 /* ------------------------------------------------------------------------ */
 
-enum dt_algorithm { DT_GRADIENT = 0, DT_GREEDY = 1 };
+enum dt_algorithm { DT_GRADIENT = 0, DT_GREEDY = 1, DT_MATCHING = 2 /* batch engine only */ };
 
 /* Termination reasons of a solve. */
 enum dt_reason {
@@ -276,7 +276,9 @@ void dt_engine_destroy(dt_engine *e);
  * in the pool too). */
 size_t dt_engine_footprint(const dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, int host_io);
 
-/* Admits a solve.  host_io = 0: target/mask/field are device buffers that
+/* Admits a solve (DT_GRADIENT, DT_GREEDY or DT_MATCHING; a matching solve
+ * reads epsilon, delta, zeta and it_max and no mask, which may be NULL).
+ * host_io = 0: target/mask/field are device buffers that
  * stay valid and untouched until the result is polled; the result is written
  * to `field`.  host_io = 1: host buffers (pinned memory lets the copies
  * overlap the passes); inputs are copied at submission and the result is
