@@ -67,6 +67,8 @@ def parse():
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
                     help="volumes of the batched-vs-plain comparison (0 to skip it)")
+    ap.add_argument("--batch-sides", default="48,160",
+                    help="range of each axis of the batch volumes (--mode batch)")
     ap.add_argument("--serve", type=int, default=100,
                     help="requests of the serving stream (0 to skip it)")
     ap.add_argument("--qps", type=float, default=80.0)
@@ -131,7 +133,7 @@ def _iterations(info):
     return info["iterations"] - (1 if info["reason"] == 6 else 0)
 
 
-def batch_bench(count, it_max, graphs, algorithm="gradient"):
+def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160)):
     """64 volumes of uneven sizes (each axis U{48..160}, spacing 1): solved one
     dt_solve_device (matching: dt_matching_device) call after the other (the
     plain one-volume-per-launch solve) and all at once by the batch engine,
@@ -141,7 +143,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient"):
     from disptools_amd import native
     from disptools_amd.workloads import batch_uneven
 
-    cases = batch_uneven(count)
+    cases = batch_uneven(count, lo=sides[0], hi=sides[1])
     dev = torch.device("cuda:0")
     vols = []
     for T, M, sp in cases:
@@ -217,8 +219,8 @@ def batch_bench(count, it_max, graphs, algorithm="gradient"):
     diffs = [abs(b["final_max_error"] - p) for b, p in zip(batch_infos, plain_max)]
     same_reason = sum(1 for b, p in zip(batch_infos, plain_infos) if b["reason"] == p["reason"])
     return {
-        "workload": "%d %s solves, each axis U{48..160}, spacing 1, smoothed log-normal target, "
-                    "ellipsoid mask, zero init, it_max=%d" % (len(vols), algorithm, it_max),
+        "workload": "%d %s solves, each axis U{%d..%d}, spacing 1, smoothed log-normal target, "
+                    "ellipsoid mask, zero init, it_max=%d" % (len(vols), algorithm, sides[0], sides[1], it_max),
         "voxels": sum(voxels),
         "plain": {"wall_s": plain_wall, "voxel_iters_per_s": plain_vi / plain_wall,
                   "iterations": sum(count_its(i) for i in plain_infos)},
@@ -409,7 +411,8 @@ def main():
         if not native.set_device(0):
             raise RuntimeError("no HIP device")
         if args.mode == "batch":
-            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm)))
+            sides = tuple(int(v) for v in args.batch_sides.split(","))
+            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides)))
         else:
             print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
                                          slots=args.slots, dump=args.serve_dump)))
