@@ -121,3 +121,29 @@ def test_engine_matching_rejects_bad_arguments(gpu_lib):
                        it_max=5, epsilon=0.0)
         with pytest.raises(gpu_lib.HipError):
             eng.submit("matching", 4, 4, 1, SPACING, T.ctypes.data, None, F.ctypes.data, host_io=True, it_max=5)
+
+
+@pytest.mark.parametrize("shape,it_max", [((2, 2, 2), 5), ((2, 9, 3), 25), ((33, 2, 5), 1), ((5, 7, 65), 0),
+                                          ((3, 3, 40), 30)])
+def test_engine_matching_edge_shapes_device_buffers(gpu_lib, shape, it_max):
+    """Two-voxel axes, it_max 0 and 1, long thin volumes; device buffers
+    (the result is written in place) and an initial field."""
+    import torch
+    print("matching edge", shape, it_max, flush=True)
+    rng = np.random.default_rng(sum(shape) + it_max)
+    T = (1.0 + 0.1 * rng.standard_normal(shape)).astype(np.float32)
+    F0 = (0.05 * rng.standard_normal((3,) + shape)).astype(np.float32)
+    params = dict(MPARAMS, it_max=it_max)
+    fr, ir, _ = gpu_lib.matching_host(T, SPACING, field=F0, **params)
+    dT = torch.from_numpy(T).to("cuda:0")
+    dF = torch.from_numpy(F0).to("cuda:0")
+    nz, ny, nx = shape
+    with gpu_lib.Engine() as eng:
+        t = eng.submit("matching", nx, ny, nz, SPACING, dT.data_ptr(), None, dF.data_ptr(), **params)
+        assert t is not None
+        st, info = _drain(eng)[t]
+    torch.cuda.synchronize()
+    assert st == 0
+    assert info["iterations"] == ir["iterations"] and info["reason"] == ir["reason"]
+    assert info["final_error"] == ir["final_cost"]
+    assert np.array_equal(dF.cpu().numpy(), fr)
