@@ -110,3 +110,17 @@ def test_sphere_config_geometry():
     T, M, sp = sphere_case(40)
     assert M.sum() == 4169
     assert set(np.unique(T)) == {np.float32(1.0), np.float32(1.1)}
+
+
+def test_algorithm_values_match_the_header():
+    """The algorithm numbers native.py passes through the C ABI are those of
+    enum dt_algorithm in include/disptools_hip.h (matching: engine only)."""
+    import re
+    from pathlib import Path
+    from disptools_amd import native
+    text = (Path(__file__).resolve().parents[1] / "include" / "disptools_hip.h").read_text()
+    body = re.search(r"enum dt_algorithm \{([^}]*)\}", text).group(1)
+    values = {k: int(v) for k, v in re.findall(r"DT_(\w+)\s*=\s*(\d+)", body)}
+    assert values == {"GRADIENT": 0, "GREEDY": 1, "MATCHING": 2}
+    assert native.ALGORITHMS == {"gradient": values["GRADIENT"], "greedy": values["GREEDY"]}
+    assert native.ENGINE_ALGORITHMS == {k.lower(): v for k, v in values.items()}
