@@ -67,6 +67,8 @@ def parse():
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
                     help="volumes of the batched-vs-plain comparison (0 to skip it)")
+    ap.add_argument("--serve-matching", type=float, default=0.0,
+                    help="fraction of the serving requests turned into matching solves (--mode serve)")
     ap.add_argument("--batch-sides", default="48,160",
                     help="range of each axis of the batch volumes (--mode batch)")
     ap.add_argument("--serve", type=int, default=100,
@@ -241,7 +243,7 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
+def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0.0):
     """Open-loop serving stream: Poisson arrivals at `qps`, cube side
     log-normal (median 96, sigma_ln 0.35, clipped to [32, 256]), 80 % gradient
     and 20 % greedy requests, targets and masks of the 128^3 workload's
@@ -258,6 +260,15 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
 
     dev = torch.device("cuda:0")
     reqs = serving_stream(count, qps, seed=seed)
+    # optionally turn a fraction of the requests into matching solves
+    # (dt_matching_device when plain, the engine's matching lane when batched)
+    mrng = np.random.default_rng(seed + 1)
+    for r in reqs:
+        if mrng.random() < matching:
+            r["algorithm"] = "matching"
+
+    def its(info, alg):
+        return info["iterations"] if alg == "matching" else _iterations(info)
     data = []
     for r in reqs:
         side = r["side"]
@@ -296,13 +307,17 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
         dT[:n].copy_(T.view(-1), non_blocking=True)
         dM[:n].copy_(M.view(-1), non_blocking=True)
         dF[:3 * n].zero_()
-        info, _ = native.solve_device(r["algorithm"], side, side, side, (1.0, 1.0, 1.0), dT.data_ptr(),
-                                      dM.data_ptr(), dF.data_ptr(), stream=sh, it_max=it_max)
+        if r["algorithm"] == "matching":
+            info, _ = native.matching_device(side, side, side, (1.0, 1.0, 1.0), dT.data_ptr(), dF.data_ptr(),
+                                             stream=sh, it_max=it_max)
+        else:
+            info, _ = native.solve_device(r["algorithm"], side, side, side, (1.0, 1.0, 1.0), dT.data_ptr(),
+                                          dM.data_ptr(), dF.data_ptr(), stream=sh, it_max=it_max)
         F.view(-1).copy_(dF[:3 * n], non_blocking=True)
         stream.synchronize()
         busy += time.perf_counter() - t_begin
         lat_plain.append(time.perf_counter() - start - r["arrival"])
-        vi_plain += n * _iterations(info)
+        vi_plain += n * its(info, r["algorithm"])
         info_plain.append(info)
     plain_span = time.perf_counter() - start - reqs[0]["arrival"]
     del dT, dM, dF
@@ -338,7 +353,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
                 i = tickets.pop(t)
                 lat_b[i] = time.perf_counter() - start - reqs[i]["arrival"]
                 info_b[i] = info
-                vi_b += voxels[i] * _iterations(info)
+                vi_b += voxels[i] * its(info, reqs[i]["algorithm"])
                 done += 1
             if left == 0 and not queue and nxt < len(reqs):
                 wait = reqs[nxt]["arrival"] - (time.perf_counter() - start)
@@ -355,8 +370,10 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump=""):
     diff = max(abs(a["final_max_error"] - b["final_max_error"]) for a, b in zip(info_plain, info_b))
     return {
         "workload": "%d requests, Poisson %.1f req/s, cube side log-normal (median 96, sigma_ln 0.35, "
-                    "clip [32, 256]), 80%% gradient / 20%% greedy, it_max=%d, host buffers (pinned)"
-                    % (len(reqs), qps, it_max),
+                    "clip [32, 256]), 80%% gradient / 20%% greedy%s, it_max=%d, host buffers (pinned)"
+                    % (len(reqs), qps, ", then %.0f%% of all turned into matching" % (100 * matching)
+                       if matching else "", it_max),
+        "matching_requests": sum(1 for r in reqs if r["algorithm"] == "matching"),
         "engine_slots": slots or 256,
         "voxels": sum(voxels),
         "plain": {"p50_s": _pct(lat_plain, 50), "p99_s": _pct(lat_plain, 99),
@@ -415,6 +432,7 @@ def main():
             print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides)))
         else:
             print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
+                                         matching=args.serve_matching,
                                          slots=args.slots, dump=args.serve_dump)))
         return
 
