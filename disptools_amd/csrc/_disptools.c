@@ -9,9 +9,11 @@
  *   - `strict` is parsed into an int ("p" into a C bool is undefined, :285,309);
  *   - the GIL is released while the device works;
  *   - None is returned with a reference (reference :166,273,421 does not INCREF);
- *   - gpu_id < 0 asks for the CPU solver, which this build does not have: it is
- *     a ValueError, the same way the reference rejects gpu_id >= 0 in a build
- *     without CUDA (:348-354).
+ *   - gpu_id < 0 (the reference's CPU branch, :374-384) calls the same
+ *     CPU-named entry points, generate_displacement_gradient / _greedy, which
+ *     in this build run on the current HIP device (there is no CPU solver);
+ *   - `matching` runs on the HIP device for any gpu_id (the reference rejects
+ *     gpu_id >= 0 for it, :368-373).
  * The `shape_descriptor` method (cubeness/octahedroness) is not part of the
  * displacement-generation path and is not provided.
  */
@@ -181,21 +183,19 @@ static PyObject *method_displacement(PyObject *self, PyObject *args)
         return NULL;
     }
 
-    if (gpu_id < 0) {
-        PyErr_Format(PyExc_ValueError,
-                     "Parameter gpu_id is set to %d, but this build of disptools runs on HIP "
-                     "devices only; pass gpu_id >= 0.", gpu_id);
-        return NULL;
-    }
-    if (!set_device(gpu_id)) {
+    /* gpu_id >= 0 selects that HIP device and the *_cuda entry points; a
+     * negative gpu_id (the reference's default, its CPU branch at
+     * _disptools.c:374-384) calls the CPU-named entry points, which run on the
+     * current HIP device: this build has no CPU solver. */
+    if (gpu_id >= 0 && !set_device(gpu_id)) {
         PyErr_Format(PyExc_ValueError, "Cannot set HIP device %d", gpu_id);
         return NULL;
     }
     displacement_fn fn = NULL;
     if (!strcmp(algorithm, ALGORITHM_GRADIENT)) {
-        fn = &generate_displacement_gradient_cuda;
+        fn = gpu_id >= 0 ? &generate_displacement_gradient_cuda : &generate_displacement_gradient;
     } else if (!strcmp(algorithm, ALGORITHM_GREEDY)) {
-        fn = &generate_displacement_greedy_cuda;
+        fn = gpu_id >= 0 ? &generate_displacement_greedy_cuda : &generate_displacement_greedy;
     } else {
         /* CPU-only in the reference (_disptools.c:368-373); here on the GPU */
         fn = &volume_matching_3d;

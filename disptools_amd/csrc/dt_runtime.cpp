@@ -930,6 +930,35 @@ extern "C" void generate_displacement_greedy_cuda(
                     beta, gamma, delta, zeta, theta, iota, strict, it_max, field);
 }
 
+// The reference's CPU-named solvers (src/headers/displacement_field_gradient.h:7-30,
+// src/headers/displacement_field_greedy.h:7-30), which its extension calls when
+// gpu_id < 0 (python_c_extension/_disptools.c:374-384).  This library has one
+// implementation, on HIP devices: these run it on the calling thread's current
+// device (device 0 unless set_device chose another), with the semantics and
+// log lines of src/displacement_field_gradient.c:191-431 and
+// src/displacement_field_greedy.c:85-247.  There is no CPU path behind them.
+extern "C" void generate_displacement_gradient(
+    const size_t nx, const size_t ny, const size_t nz, const float dx, const float dy, const float dz,
+    const float *J, const bool *mask, const float epsilon, const float tolerance, float eta,
+    const float eta_max, const float alpha, const float beta, const float gamma, const float delta,
+    const float zeta, const float theta, const float iota, const bool strict, const size_t it_max,
+    float *field)
+{
+    reference_entry(DT_GRADIENT, nx, ny, nz, dx, dy, dz, J, mask, epsilon, tolerance, eta, eta_max, alpha,
+                    beta, gamma, delta, zeta, theta, iota, strict, it_max, field);
+}
+
+extern "C" void generate_displacement_greedy(
+    const size_t nx, const size_t ny, const size_t nz, const float dx, const float dy, const float dz,
+    const float *J, const bool *mask, const float epsilon, const float tolerance, float eta,
+    const float eta_max, const float alpha, const float beta, const float gamma, const float delta,
+    const float zeta, const float theta, const float iota, const bool strict, const size_t it_max,
+    float *field)
+{
+    reference_entry(DT_GREEDY, nx, ny, nz, dx, dy, dz, J, mask, epsilon, tolerance, eta, eta_max, alpha,
+                    beta, gamma, delta, zeta, theta, iota, strict, it_max, field);
+}
+
 extern "C" int dt_jacobian_device(size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
                                   const float *d_field, float *d_J, void *stream)
 {

@@ -9,9 +9,10 @@ given by the keyword ``spacing`` (default 1).
 
 Behaviour of this build that differs from the reference:
 
-* the solve runs on a HIP device; ``gpu_id`` must be >= 0 (``-1`` selects the
-  reference's CPU solver, which this build does not have, and raises
-  ``ValueError`` from the extension);
+* the solve always runs on a HIP device: ``gpu_id >= 0`` selects the device,
+  the default ``gpu_id=-1`` (the reference's CPU solver) runs on the current
+  HIP device through the reference's CPU-named entry points
+  (``generate_displacement_gradient`` / ``_greedy``); there is no CPU solver;
 * the image pyramid (``levels > 1``, and an ``initial_guess``, which the
   reference also passes through ``drawing.scale_image``) is built on the
   device by ``dt_scale_volume_device``: a Gaussian of sigma 2 (physical
@@ -174,11 +175,11 @@ def _pyramid_displacement(J, M, guess, spacing, levels, it_max, parameters):
     import torch
     from . import native
     gpu_id = int(parameters["gpu_id"])
-    if gpu_id < 0:
-        raise ValueError("gpu_id must select a HIP device")
-    if not native.set_device(gpu_id):
+    if gpu_id >= 0 and not native.set_device(gpu_id):
         raise ValueError("HIP device %d is not available" % gpu_id)
-    dev = torch.device("cuda", gpu_id)
+    # gpu_id < 0: the current device (the reference's CPU solver, which this
+    # build does not have, runs there instead)
+    dev = torch.device("cuda", gpu_id if gpu_id >= 0 else torch.cuda.current_device())
     nz, ny, nx = J.shape
     sizes = pyramid_sizes((nx, ny, nz), levels)
     dJ = torch.from_numpy(np.ascontiguousarray(J, np.float32)[None]).to(dev)

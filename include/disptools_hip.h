@@ -72,6 +72,31 @@ void generate_displacement_greedy_cuda(
         const float zeta, const float theta, const float iota,
         const bool strict, const size_t it_max, float *field);
 
+/* Replaces the CPU-named solvers generate_displacement_gradient
+ * (src/headers/displacement_field_gradient.h:7-30) and
+ * generate_displacement_greedy (src/headers/displacement_field_greedy.h:7-30),
+ * which the reference extension calls for gpu_id < 0
+ * (python_c_extension/_disptools.c:374-384).  Same arguments, semantics and
+ * log lines; they run on the calling thread's current HIP device (this
+ * library has no CPU solver). */
+void generate_displacement_gradient(
+        const size_t nx, const size_t ny, const size_t nz,
+        const float dx, const float dy, const float dz,
+        const float *J, const bool *mask,
+        const float epsilon, const float tolerance, float eta, const float eta_max,
+        const float alpha, const float beta, const float gamma, const float delta,
+        const float zeta, const float theta, const float iota,
+        const bool strict, const size_t it_max, float *field);
+
+void generate_displacement_greedy(
+        const size_t nx, const size_t ny, const size_t nz,
+        const float dx, const float dy, const float dz,
+        const float *J, const bool *mask,
+        const float epsilon, const float tolerance, float eta, const float eta_max,
+        const float alpha, const float beta, const float gamma, const float delta,
+        const float zeta, const float theta, const float iota,
+        const bool strict, const size_t it_max, float *field);
+
 /* Replaces jacobian_dynamic (src/headers/jacobian.h:24-33, src/jacobian.c:7-21)
  * as bound by _disptools.jacobian (python_c_extension/_disptools.c:238-274):
  * f is [3][nz][ny][nx], J is [nz][ny][nx]; computed on the current device. */

@@ -3,6 +3,10 @@ symbol include/disptools_hip.h declares; the extension has the reference's
 method table and error behaviour on the paths that fail before any device work."""
 
 import ctypes
+import os
+import shutil
+import subprocess
+import sysconfig
 
 import numpy as np
 import pytest
@@ -12,7 +16,8 @@ def test_library_exports_every_declared_symbol():
     from disptools_amd import native
     names = native.declared_symbols()
     for required in ("set_device", "generate_displacement_gradient_cuda",
-                     "generate_displacement_greedy_cuda", "jacobian_dynamic", "regularise",
+                     "generate_displacement_greedy_cuda", "generate_displacement_gradient",
+                     "generate_displacement_greedy", "jacobian_dynamic", "regularise",
                      "disptools_get_error", "disptools_has_error", "disptools_clear_error",
                      "get_float_type_size", "dt_solve_device", "dt_solve_host", "dt_jacobian_device"):
         assert required in names
@@ -64,8 +69,6 @@ def test_extension_error_behaviour():
     ext = load_extension()
     with pytest.raises(ValueError, match="Invalid algorithm"):
         ext.displacement(*_disp_args(algorithm="fancy"))
-    with pytest.raises(ValueError, match="gpu_id"):
-        ext.displacement(*_disp_args(gpu_id=-1))
     with pytest.raises(ValueError, match="dtype"):
         ext.displacement(*_disp_args(J=np.ones((6, 6, 6), np.float64)))
     with pytest.raises(ValueError, match="mask"):
@@ -76,13 +79,56 @@ def test_extension_error_behaviour():
         ext.displacement(*_disp_args(field=np.zeros((3, 6, 6, 5), np.float32)))
 
 
-def test_extension_rejects_matching_on_device():
+def test_extension_default_gpu_id_reaches_the_library():
+    """gpu_id = -1 (the reference default, its CPU branch) is not rejected by
+    the extension: it calls generate_displacement_gradient, which runs on the
+    current HIP device.  Without a device the library reports its own error
+    (RuntimeError), never the ValueError of a rejected argument."""
     from disptools_amd import load_extension, native
     ext = load_extension()
-    if not native.set_device(0):
-        pytest.skip("no HIP device: the device check fails first")
-    with pytest.raises(ValueError, match="not supported on GPU"):
-        ext.displacement(*_disp_args(algorithm="matching"))
+    if native.set_device(0):
+        pytest.skip("a HIP device is present: the GPU tests cover this call")
+    with pytest.raises(RuntimeError):
+        ext.displacement(*_disp_args(gpu_id=-1))
+
+
+def _reference_extension_imports(tmp_path, cuda):
+    """Undefined symbols of the reference extension's object file
+    (python_c_extension/_disptools.c compiled from /root/reference with or
+    without its CUDA branch)."""
+    src = "/root/reference/python_c_extension/_disptools.c"
+    if not os.path.exists(src) or shutil.which("gcc") is None or shutil.which("nm") is None:
+        pytest.skip("reference sources or toolchain not present")
+    obj = tmp_path / ("ext_%d.o" % cuda)
+    subprocess.run(["gcc", "-std=c99", "-c", "-fPIC", "-w", "-DFLOAT_SIZE=32", "-DDISPTOOLS_HAS_CUDA=%d" % cuda,
+                    "-I/root/reference/src/headers", "-I/root/reference/cuda/headers",
+                    "-I" + sysconfig.get_paths()["include"], "-I" + np.get_include(), src, "-o", str(obj)],
+                   check=True)
+    out = subprocess.run(["nm", "-u", str(obj)], check=True, capture_output=True, text=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+@pytest.mark.parametrize("cuda", [0, 1])
+def test_reference_extension_links_against_the_library(tmp_path, cuda):
+    """Every library symbol the reference's own _disptools.c references
+    (:13-21 includes, :347-386 dispatch, :138-274 jacobian/regularise) is
+    exported by libdisptools_hip.so, with and without its CUDA branch -- except
+    the shape descriptors (cubeness, octahedroness), which are not on the
+    displacement-generation path."""
+    from disptools_amd import native
+    wanted = {s for s in _reference_extension_imports(tmp_path, cuda)
+              if not s.startswith(("Py", "_Py", "_GLOBAL_OFFSET", "__"))
+              and s not in ("strcmp", "memcpy", "memset", "strlen")}
+    out_of_scope = {"cubeness", "octahedroness"}
+    assert {"generate_displacement_gradient", "generate_displacement_greedy", "jacobian_dynamic",
+            "regularise", "disptools_get_error", "disptools_has_error"} <= wanted
+    if cuda:
+        assert {"set_device", "generate_displacement_gradient_cuda", "generate_displacement_greedy_cuda"} <= wanted
+    exported = subprocess.run(["nm", "-D", "--defined-only", native.LIB_PATH], check=True, capture_output=True,
+                              text=True).stdout
+    exported = {line.split()[-1] for line in exported.splitlines() if line.strip()}
+    missing = sorted(wanted - out_of_scope - exported)
+    assert not missing, missing
 
 
 def test_pyramid_schedule():
