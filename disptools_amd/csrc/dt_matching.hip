@@ -901,6 +901,12 @@ int matching_validate(size_t nx, size_t ny, size_t nz, float dx, float dy, float
 {
     if (nx < 2 || ny < 2 || nz < 2) DT_FAIL("every image dimension must be at least 2 voxels");
     if (nx > (1u << 30) || ny > (1u << 30) || nz > (1u << 30)) DT_FAIL("image too large");
+    // voxel count and the byte counts derived from it (16 B/voxel of staging,
+    // 40 B/voxel of solver state) in checked arithmetic: no size_t product wraps
+    size_t n = 0, bytes = 0;
+    if (__builtin_mul_overflow(nx, ny, &n) || __builtin_mul_overflow(n, nz, &n) ||
+        __builtin_mul_overflow(n, (size_t)64, &bytes) || n > ((size_t)1 << 40))
+        DT_FAIL("image too large: at most 2^40 voxels per matching solve");
     if (!(dx > 0.0f) || !(dy > 0.0f) || !(dz > 0.0f)) DT_FAIL("spacing must be positive");
     if (!(epsilon > 0.0f)) DT_FAIL("epsilon must be positive");
     if (!(delta > 0.0f) || !(zeta > 0.0f)) DT_FAIL("delta and zeta must be positive");

@@ -60,6 +60,27 @@ class SolveResponse:
     seconds: float = 0.0
 
 
+# Smallest axis each algorithm accepts: the Jacobian stencils read 3 samples
+# per axis (src/headers/jacobian_macros.h:8-99); the matching cells span 2.
+MIN_AXIS = {"gradient": 3, "greedy": 3, "matching": 2}
+
+
+def check_request(shape, algorithm, params):
+    """Rejects a request the solvers would refuse (ValueError), before it is
+    queued: unknown algorithm or parameters, axes below the algorithm's
+    minimum, a non-positive matching epsilon."""
+    if algorithm not in ALGORITHMS:
+        raise ValueError("unknown algorithm %r" % (algorithm,))
+    unknown = set(params) - set(PARAM_NAMES)
+    if unknown:
+        raise ValueError("unknown solver parameters: %s" % sorted(unknown))
+    if len(shape) != 3 or min(shape) < MIN_AXIS[algorithm]:
+        raise ValueError("%s needs every axis of at least %d voxels, got shape %r"
+                         % (algorithm, MIN_AXIS[algorithm], tuple(shape)))
+    if algorithm == "matching" and "epsilon" in params and not float(params["epsilon"]) > 0.0:
+        raise ValueError("matching needs epsilon > 0")
+
+
 def _split(buf, magic):
     mv = memoryview(buf)
     if bytes(mv[:len(magic)]) != magic:
@@ -118,11 +139,7 @@ def decode_request(buf) -> SolveRequest:
     if head.get("has_field"):
         f, off = _take(mv, off, 12 * n, "<f4", (3,) + shape)
     params = dict(head.get("params", {}))
-    unknown = set(params) - set(PARAM_NAMES)
-    if unknown:
-        raise ValueError("unknown solver parameters: %s" % sorted(unknown))
-    if head.get("algorithm") not in ALGORITHMS:
-        raise ValueError("unknown algorithm %r" % (head.get("algorithm"),))
+    check_request(shape, head.get("algorithm"), params)
     return SolveRequest(target=t.astype(np.float32), mask=m.astype(bool), spacing=tuple(head["spacing"]),
                         algorithm=head["algorithm"], params=params,
                         field=None if f is None else f.astype(np.float32), request_id=head["id"])
@@ -169,6 +186,7 @@ class SolveServer:
         self.engine = native.Engine(hbm_budget=hbm_budget, max_slots=max_slots, **engine_kw)
         self._queue: List[Tuple[SolveRequest, tuple]] = []
         self._live: Dict[int, Tuple[SolveRequest, tuple]] = {}
+        self._failed: List[SolveResponse] = []
 
     def close(self):
         self.engine.close()
@@ -180,11 +198,10 @@ class SolveServer:
         self.close()
 
     def pending(self):
-        return len(self._queue) + len(self._live)
+        return len(self._queue) + len(self._live) + len(self._failed)
 
     def submit(self, req: SolveRequest):
-        if req.algorithm not in ALGORITHMS:
-            raise ValueError("algorithm must be one of %s" % (ALGORITHMS,))
+        check_request(np.shape(req.target), req.algorithm, req.params)
         torch = self._torch
         t = torch.from_numpy(np.ascontiguousarray(req.target, np.float32)).pin_memory()
         m = torch.from_numpy(np.ascontiguousarray(req.mask, np.uint8)).pin_memory()
@@ -199,15 +216,23 @@ class SolveServer:
         while self._queue:
             req, (t, m, f) = self._queue[0]
             nz, ny, nx = t.shape
-            ticket = self.engine.submit(req.algorithm, nx, ny, nz, req.spacing, t.data_ptr(), m.data_ptr(),
-                                        f.data_ptr(), host_io=True, **req.params)
+            try:
+                ticket = self.engine.submit(req.algorithm, nx, ny, nz, req.spacing, t.data_ptr(),
+                                            m.data_ptr(), f.data_ptr(), host_io=True, **req.params)
+            except self._native.HipError as e:
+                # the engine refused this request (e.g. larger than the pool):
+                # answer it as failed and go on with the others
+                self._queue.pop(0)
+                self._failed.append(SolveResponse(request_id=req.request_id, status=-1, field=f.numpy().copy(),
+                                                  reason=0, reason_text=str(e)))
+                continue
             if ticket is None:
                 return
             self._live[ticket] = self._queue.pop(0)
 
     def step(self, wait=True) -> List[SolveResponse]:
         self._admit()
-        out = []
+        out, self._failed = self._failed, []
         if self._live:
             self.engine.progress(wait=wait)
         for ticket, status, info in self.engine.poll():
@@ -218,6 +243,8 @@ class SolveServer:
                                      rejections=info["rejections"], final_error=info["final_error"],
                                      final_max_error=info["final_max_error"], seconds=info["seconds"]))
         self._admit()
+        out.extend(self._failed)
+        self._failed = []
         return out
 
     def run(self, requests) -> List[SolveResponse]:

@@ -69,3 +69,32 @@ def test_matching_request_and_unknown_algorithm():
     bad = encode_request(_request(False)).replace(b'"algorithm": "greedy"', b'"algorithm": "nope"')
     with pytest.raises(ValueError):
         decode_request(bad)
+
+
+def test_requests_the_solvers_refuse_are_rejected_before_queueing():
+    """Axes below the algorithm's minimum (3 for the Jacobian stencils, 2 for
+    matching cells) and a non-positive matching epsilon are refused when the
+    request is decoded, so a bad request never reaches the engine queue."""
+    from disptools_amd.serving import SolveRequest, check_request, decode_request, encode_request
+    t = np.ones((2, 8, 8), np.float32)
+    buf = encode_request(SolveRequest(target=t, mask=t > 0, algorithm="gradient"))
+    with pytest.raises(ValueError, match="at least 3"):
+        decode_request(buf)
+    decode_request(encode_request(SolveRequest(target=t, mask=t > 0, algorithm="matching")))
+    with pytest.raises(ValueError, match="at least 2"):
+        check_request((1, 8, 8), "matching", {})
+    with pytest.raises(ValueError, match="epsilon"):
+        check_request((4, 4, 4), "matching", {"epsilon": 0.0})
+    check_request((3, 3, 3), "greedy", {"epsilon": 0.0})
+
+
+def test_matching_volume_size_is_checked_without_wrapping():
+    """dt_matching_host rejects a volume whose voxel count would wrap a size_t
+    product, before any device work."""
+    import ctypes
+    from disptools_amd import native
+    info = native.MatchingInfo()
+    big = 1 << 22
+    rc = native.lib.dt_matching_host(big, big, big, 1.0, 1.0, 1.0, None, 1e-3, 1e-3, 10.0, 10, None,
+                                     ctypes.byref(info), None, 0)
+    assert rc == -1 and "too large" in native.last_error()
