@@ -42,13 +42,47 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel-iters/s at 128³ + p50/p99 solve latency @ fixed QPS vs plain GPU solve"
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per k_grad_stream launch of the default solve (128^3, it_max 2000,
-# default pass variant: 64 x 12 column tiles), from rocprofv3 --pmc FETCH_SIZE
-# and --pmc WRITE_SIZE runs of this script, scaled by what the counters read
-# for a 1 GiB copy (--mode copy): read bytes = 2.0 x FETCH_SIZE, written =
-# 1.0 x WRITE_SIZE; the mean over all 42 launches of a solve (40 evaluated
-# passes and 2 idle launches).
-MEASURED_TRAFFIC_128 = 86.76e6
+# HBM bytes per k_grad_stream launch of the default solve come from the
+# committed rocprofv3 summary of that kernel (its --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script, scaled by what the counters read for a
+# 1 GiB copy): the line "bench-traffic-bytes-per-launch: N" there.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_grad_stream_128cube.md")
+
+
+def measured_traffic():
+    """(bytes per launch, source) from the kernel's committed profile, or (None, reason)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith("bench-traffic-bytes-per-launch:"):
+                    return float(line.split(":", 1)[1]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    except OSError:
+        pass
+    return None, "no measured traffic in %s" % os.path.relpath(TRAFFIC_PROFILE, ROOT)
+
+
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (affinity mask
+    and `lscpu -p=CPU,CORE,SOCKET`); the affinity count if lscpu is missing."""
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        cores = set()
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(v) if v else 0 for v in line.split(",")[:3])
+            if cpu in allowed:
+                cores.add((sock, core))
+        if cores:
+            return len(cores), "lscpu: %d physical cores among %d allowed CPUs" % (len(cores), len(allowed))
+    except Exception:
+        pass
+    return len(allowed), "%d allowed CPUs (lscpu unavailable)" % len(allowed)
 
 
 def parse():
@@ -145,7 +179,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160)):
     from disptools_amd import native
     from disptools_amd.workloads import batch_uneven
 
-    cases = batch_uneven(count, lo=sides[0], hi=sides[1])
+    cases = batch_uneven(count, lo=sides[0], hi=sides[1], workers=8)
     dev = torch.device("cuda:0")
     vols = []
     for T, M, sp in cases:
@@ -512,15 +546,22 @@ def main():
                            dump=args.serve_dump)
                if args.serve > 0 else None)
 
-    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    cores, cores_source = physical_cores()
+    threads = args.cpu_threads or cores
     # the reference's matching is single-threaded and takes seconds per
     # iteration at 128^3: its sample is a few iterations
     cpu_iters = min(args.cpu_iters, 5) if args.algorithm == "matching" else args.cpu_iters
-    cpu = (cpu_baseline(T, M, sp, cpu_iters, threads, algorithm=args.algorithm)
-           if args.cpu_iters > 0 else None)
+    cpu = None
+    if args.cpu_iters > 0:
+        cpu = cpu_baseline(T, M, sp, cpu_iters, threads, algorithm=args.algorithm)
+        cpu["cores_source"] = cores_source if not args.cpu_threads else "--cpu-threads"
+        if threads > 1 and args.algorithm != "matching":
+            one = cpu_baseline(T, M, sp, cpu_iters, 1, algorithm=args.algorithm, max_repeats=2)
+            cpu["one_thread"] = {"value": one["value"], "unit": one["unit"], "sample": one["sample"]}
 
     default_workload = (args.it_max == 2000 and args.algorithm == "gradient" and
                         "DISPTOOLS_GRAD_VARIANT" not in os.environ)
+    traffic, traffic_source = measured_traffic() if default_workload else (None, "not the profiled workload")
     last = infos[-1]
     out = {
         "metric": METRIC,
@@ -540,7 +581,7 @@ def main():
                    "voxels": n, "it_max": args.it_max},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": MEASURED_TRAFFIC_128 if default_workload else None,
+                     "traffic": traffic, "traffic_source": traffic_source,
                      "kernel": {"gradient": "k_grad_stream", "greedy": "k_greedy_stream",
                                 "matching": "k_match_pass"}[args.algorithm],
                      "avg_launch_us": avg_launch_s * 1e6,

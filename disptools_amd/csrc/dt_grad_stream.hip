@@ -14,11 +14,18 @@
 //    three of them;
 //  * a thread's first three load slots are its primary point's components,
 //    so the z-neighbours of its point come from registers;
-//  * LDS holds two trial planes (halo 3) and two planes of W_{a,x} / W_{a,y}
-//    (halo 1 along their own axis), indexed by plane parity: one barrier per
-//    plane separates every write from its reads;
+//  * LDS holds a ring of three trial planes (halo 3) and two planes of
+//    W_{a,x} / W_{a,y} (halo 1 along their own axis), indexed by plane
+//    parity: one barrier per plane separates every write from its reads;
 //  * the rest of the halo (rings 2 and 3, ring-1 corners) is spread over all
-//    threads.
+//    threads;
+//  * no register that a load writes is carried across the loop's back edge:
+//    a plane's loads are resolved and committed to the LDS ring at the top of
+//    the next step (the ring's third plane is what lets the commit happen at
+//    once), and target / mask alternate between two register sets (the loop
+//    body is two steps).  A loaded register copied at the back edge makes the
+//    step wait for the load it has just issued, which turns the one-plane
+//    prefetch into a full memory round trip per step.
 #include "dt_device.hpp"
 #include "dt_kernels.hpp"
 #include "dt_pass_common.hpp"
@@ -41,7 +48,7 @@ struct StreamGeom {
 template <int TX, int TY>
 struct StreamSmem {
     using G = StreamGeom<TX, TY>;
-    float xs[2][3][G::PL];            // trial planes by parity, halo 3
+    float xs[3][3][G::PL];            // ring of three trial planes, halo 3
     float wx[2][3][TY][TX + 2];       // W_{a,x} by parity, x-halo 1
     float wy[2][3][TY + 2][TX];       // W_{a,y} by parity, y-halo 1
     double red[3 * (G::NT / 64)];
@@ -209,8 +216,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             o.v[k] = ok ? v : 0.0f;
         }
     };
-    auto commit = [&](int gz, const Val &o) {
-        float *dst = &S.xs[gz & 1][0][0];
+    auto commit = [&](int slot, const Val &o) {
+        float *dst = &S.xs[slot][0][0];
         if (pi >= 0) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) dst[c * PL + pi] = o.v[c];
@@ -228,29 +235,34 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 
     // z-window of the primary point: planes zz-2 .. zz+2 at stage A of plane zz
     float col[5][3];
-    Raw rn;          // raw loads of plane zz+2 in flight at step zz
-    Val vn;          // resolved plane zz+1 (committed at step zz)
-    TM tm;           // target / mask of plane zz
+    Raw rn;          // raw loads of plane zz+2 in flight at the start of step zz
+    TM tma, tmb;     // target / mask: plane zz in one set, plane zz+1 loading into the other
 
-    // prologue: planes z0-3 .. z0 resolved, plane z0-1 committed
+    // Steps run from zs to z1 in pairs: zs = z0 - 1, or z0 - 2 when that count
+    // is odd (the extra first step's results are never used: stage A below z0
+    // only feeds stage B of planes below z0, which is skipped).
+    const int zs = ((z1 - z0) & 1) ? z0 - 2 : z0 - 1;
+    // LDS ring slot of plane z: (z - zs) mod 3
+    // prologue: planes zs and zs+1 committed (slots 0 and 1), plane zs+2 in flight
     {
         Raw r0, r1;
-        fetch(z0 - 3, r0);
-        fetch(z0 - 2, r1);
-        fetch_tm(z0 - 1, tm);
+        fetch(zs - 2, r0);
+        fetch(zs - 1, r1);
+        fetch_tm(zs, tma);
         Val v;
-        resolve(z0 - 3, r0, v);
+        resolve(zs - 2, r0, v);
         col[1][0] = v.v[0]; col[1][1] = v.v[1]; col[1][2] = v.v[2];
-        fetch(z0 - 1, r0);
-        resolve(z0 - 2, r1, v);
+        fetch(zs, r0);
+        resolve(zs - 1, r1, v);
         col[2][0] = v.v[0]; col[2][1] = v.v[1]; col[2][2] = v.v[2];
-        fetch(z0, r1);
-        resolve(z0 - 1, r0, v);
+        fetch(zs + 1, r1);
+        resolve(zs, r0, v);
         col[3][0] = v.v[0]; col[3][1] = v.v[1]; col[3][2] = v.v[2];
-        commit(z0 - 1, v);
-        fetch(z0 + 1, rn);
-        resolve(z0, r1, vn);
-        col[4][0] = vn.v[0]; col[4][1] = vn.v[1]; col[4][2] = vn.v[2];
+        commit(0, v);
+        fetch(zs + 2, rn);
+        resolve(zs + 1, r1, v);
+        col[4][0] = v.v[0]; col[4][1] = v.v[1]; col[4][2] = v.v[2];
+        commit(1, v);
         col[0][0] = col[0][1] = col[0][2] = 0.0f;
     }
     __syncthreads();
@@ -258,24 +270,63 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     CT wz_m[3] = {0, 0, 0}, wz_c[3] = {0, 0, 0};
     const CT cix = (CT)P.inv[0], ciy = (CT)P.inv[1], ciz = (CT)P.inv[2];
 
-    // step zz: stage A on plane zz, stage B on plane zz-1, commit plane zz+1
-    auto step = [&](int zz) {
-        // plane zz+2 (loads issued one step ago) joins the window
-        Val vz1;
-        resolve(zz + 2, rn, vz1);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            col[0][c] = col[1][c]; col[1][c] = col[2][c]; col[2][c] = col[3][c]; col[3][c] = col[4][c];
-            col[4][c] = vz1.v[c];
+    // Stage B's outputs (speculative candidate, and the trial itself after a
+    // rejection) are stored at the start of the next step, before that step's
+    // loads: the waits for a step's loads then find the stores long done
+    // (loads and stores share one counter, which the compiler can only wait
+    // to zero when both kinds are pending).
+    // (the trial values of the pending plane are window slot 0 by then)
+    struct Pending { bool on; unsigned so; float n[3]; };
+    Pending pend;
+    pend.on = false;
+    pend.so = 0u;
+    auto store_pending = [&]() {
+        if (pend.on) {
+            if (has_next) {
+                bstore(r_next, pend.n[0], pv, pend.so);
+                bstore(r_next, pend.n[1], pv, pend.so + cn4);
+                bstore(r_next, pend.n[2], pv, pend.so + 2u * cn4);
+            }
+            if (has_trial) {
+                bstore(r_trial, col[0][0], pv, pend.so);
+                bstore(r_trial, col[0][1], pv, pend.so + cn4);
+                bstore(r_trial, col[0][2], pv, pend.so + 2u * cn4);
+            }
         }
+        pend.on = false;
+    };
+
+    // step zz: plane zz+2 joins the window and the LDS ring, stage A on plane
+    // zz (ring slot sa), stage B on plane zz-1
+    auto step = [&](int zz, int sa, TM &cur, TM &nxt) {
+        // The loads of plane zz+2 and of this plane's target / mask were issued
+        // one step ago.  An empty asm that takes them here keeps the compiler
+        // from hoisting their first use above the barrier that ended the last
+        // step, which would make that step wait for its own loads.
+#pragma unroll
+        for (int k = 0; k < SLOTS; ++k) {
+            asm volatile("" : "+v"(rn.a[k]));
+            if (INTERP) asm volatile("" : "+v"(rn.b[k]));
+        }
+        asm volatile("" : "+v"(cur.t));
+        {
+            Val vz;
+            resolve(zz + 2, rn, vz);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                col[0][c] = col[1][c]; col[1][c] = col[2][c]; col[2][c] = col[3][c]; col[3][c] = col[4][c];
+                col[4][c] = vz.v[c];
+            }
+            commit(sa == 0 ? 2 : sa - 1, vz);
+        }
+        store_pending();
         fetch(zz + 3, rn);
-        const TM cur = tm;
-        fetch_tm(zz + 1, tm);
+        fetch_tm(zz + 1, nxt);
 
         // (A) plane zz at the primary point
         CT wz_p[3] = {0, 0, 0};
         if (zz >= 0 && zz < nz && pin) {
-            const float *R0 = &S.xs[zz & 1][0][0] + pi;
+            const float *R0 = &S.xs[sa][0][0] + pi;
             float sm[3][3][4];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -303,6 +354,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
+        pend.so = 4u * (unsigned)max(zb, 0) * plane;   // wave-uniform (a scalar offset of the stores)
         if (own && pin && zb >= z0 && zb < z1) {
             const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
@@ -318,32 +370,32 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                 f2 = (float)(cix * dzx - ciy * dzy + ciz * (wz_m[2] - wz_p[2]));
                 Gsum += (double)f0 * f0 + (double)f1 * f1 + (double)f2 * f2;
             }
-            // trial value of this voxel on plane zb: window slot 1 (planes zz-2 .. zz+2)
+            // trial value of this voxel on plane zb: window slot 1 (planes zz-2 .. zz+2);
+            // stored at the start of the next step
             const float ux = col[1][0], uy = col[1][1], uz = col[1][2];
-            const unsigned so = 4u * (unsigned)zb * plane;
-            if (has_next) {
-                bstore(r_next, ux - eta_next * f0, pv, so);
-                bstore(r_next, uy - eta_next * f1, pv, so + cn4);
-                bstore(r_next, uz - eta_next * f2, pv, so + 2u * cn4);
-            }
-            if (has_trial) {
-                bstore(r_trial, ux, pv, so);
-                bstore(r_trial, uy, pv, so + cn4);
-                bstore(r_trial, uz, pv, so + 2u * cn4);
-            }
+            pend.on = true;
+            pend.n[0] = ux - eta_next * f0; pend.n[1] = uy - eta_next * f1; pend.n[2] = uz - eta_next * f2;
         }
 #pragma unroll
         for (int a = 0; a < 3; ++a) { wz_m[a] = wz_c[a]; wz_c[a] = wz_p[a]; }
-
-        // plane zz+1 into LDS for the next step
-        commit(zz + 1, vn);
-        vn = vz1;
         __syncthreads();
     };
 
-    // not unrolled: two steps in flight at once would double the live registers
+    // two steps per trip, so the target / mask sets swap roles without a copy;
+    // the trip is not unrolled further (more steps in flight would raise the
+    // live registers)
+    int sa = 0;
 #pragma unroll 1
-    for (int zz = z0 - 1; zz <= z1; ++zz) step(zz);
+    for (int zz = zs; zz < z1; zz += 2) {
+        step(zz, sa, tma, tmb);
+        sa = sa == 2 ? 0 : sa + 1;
+        step(zz + 1, sa, tmb, tma);
+        sa = sa == 2 ? 0 : sa + 1;
+    }
+    // the last step's outputs: shift the window as a next step would
+#pragma unroll
+    for (int c = 0; c < 3; ++c) col[0][c] = col[1][c];
+    store_pending();
 }
 
 template <typename CT, int TX, int TY, int MINW>

@@ -82,14 +82,21 @@ def volume_128(seed=SEED_128):
     return target, mask, (1.0, 1.0, 1.5)
 
 
-def batch_uneven(count=64, seed=2024, lo=48, hi=160):
+def batch_uneven(count=64, seed=2024, lo=48, hi=160, workers=1):
+    """Volumes of the batch configuration; `workers` threads build them (each
+    volume has its own seed, so the result does not depend on `workers`)."""
     rng = np.random.default_rng(seed)
-    cases = []
-    for i in range(count):
-        shape = tuple(int(v) for v in rng.integers(lo, hi + 1, size=3))
-        target, mask = volume_case(shape, seed * 1000 + i)
-        cases.append((target, mask, (1.0, 1.0, 1.0)))
-    return cases
+    shapes = [tuple(int(v) for v in rng.integers(lo, hi + 1, size=3)) for _ in range(count)]
+
+    def one(i):
+        target, mask = volume_case(shapes[i], seed * 1000 + i)
+        return target, mask, (1.0, 1.0, 1.0)
+
+    if workers <= 1:
+        return [one(i) for i in range(count)]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(workers) as ex:
+        return list(ex.map(one, range(count)))
 
 
 def precision_256():

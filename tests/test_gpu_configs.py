@@ -1,0 +1,206 @@
+"""The benchmark configurations at their own sizes, against the oracle.
+
+* 128^3 volume (spacing (1, 1, 1.5), smoothed log-normal target, ellipsoid
+  mask, it_max 2000), gradient and greedy: the same termination reason as the
+  FP64 oracle and as the FP32 oracle (the reference's release numerics,
+  bit-identical to the reference C at one thread), an iteration count within
+  10 % of the FP64 oracle's, and a final masked max |J - T| within 1e-4 of it.
+* The 64-volume batch (each axis U{48..160}): every engine result is
+  bit-identical to the lone dt_solve_device solve of the same volume (same
+  column tile, lone-solve z-chunks), and a seeded subset of 8 volumes --
+  the two smallest, the two largest and four drawn at random -- holds the
+  parity bar against the FP64 oracle.
+* A 30-request serving stream (Poisson arrivals, log-normal cube sides,
+  80 % gradient / 20 % greedy) through SolveServer: every response equals the
+  lone solve of its request, and a subset of 8 holds the parity bar against
+  the FP64 oracle.
+The oracle runs on up to 16 host threads here (FP64 sums: the thread count
+moves them by rounding only; the termination reasons checked are the same at
+1 and 8 threads on these inputs).
+"""
+
+import contextlib
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MAX_ERR_TOL = 1e-4
+SAME_KEYS = ("iterations", "evaluations", "rejections", "reason", "final_error", "final_max_error", "field_io")
+
+
+def _threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+@contextlib.contextmanager
+def _oracle_threads(*oracles):
+    for o in oracles:
+        o.set_threads(_threads())
+    try:
+        yield
+    finally:
+        for o in oracles:
+            o.set_threads(1)
+
+
+def _masked_max(oracle32, field, T, M, sp):
+    _, mx, _ = oracle32.compute_error(T, oracle32.jacobian(field, sp), M, 0.2)
+    return mx
+
+
+def _check_against_fp64(oracle32, oracle64, alg, T, M, sp, it_max, field, info, tag):
+    fo, io = oracle64.solve(alg, T.astype(np.float64), M, sp, it_max=it_max, trace_cap=0)
+    mo = _masked_max(oracle32, fo.astype(np.float32), T, M, sp)
+    mg = _masked_max(oracle32, field, T, M, sp)
+    print("%s %s %s: reason %d (fp64 oracle %d), it %d (%d), max err %.7g (%.7g)"
+          % (tag, alg, T.shape, info["reason"], io["reason"], info["iterations"], io["iterations"], mg, mo),
+          flush=True)
+    assert info["reason"] == io["reason"], (tag, info["reason"], io["reason"])
+    assert abs(mg - mo) <= MAX_ERR_TOL, (tag, mg, mo)
+    assert abs(info["iterations"] - io["iterations"]) <= max(3, 0.1 * io["iterations"]), \
+        (tag, info["iterations"], io["iterations"])
+    return io
+
+
+@pytest.mark.parametrize("alg", ["gradient", "greedy"])
+def test_volume_128_against_oracle(gpu_lib, oracle32, oracle64, alg):
+    from disptools_amd.workloads import volume_128
+    T, M, sp = volume_128()
+    print("128^3", alg, flush=True)
+    f, info, _ = gpu_lib.solve_host(alg, T, M, sp, it_max=2000)
+    with _oracle_threads(oracle32, oracle64):
+        io = _check_against_fp64(oracle32, oracle64, alg, T, M, sp, 2000, f, info, "128^3")
+        _, i32 = oracle32.solve(alg, T, M, sp, it_max=2000, trace_cap=0)
+    assert info["reason"] == i32["reason"] == io["reason"]
+    assert info["final_max_error"] == pytest.approx(_masked_max(oracle32, f, T, M, sp), rel=0, abs=5e-7)
+
+
+class _DevVolume:
+    def __init__(self, T, M):
+        import torch
+        d = torch.device("cuda:0")
+        self.T = torch.from_numpy(T).to(d)
+        self.M = torch.from_numpy(M.astype(np.uint8)).to(d)
+        self.F = torch.zeros((3,) + T.shape, dtype=torch.float32, device=d)
+
+
+def _oracle_subset(sizes, k_random=4, seed=99):
+    order = np.argsort(sizes, kind="stable")
+    pick = [int(order[0]), int(order[1]), int(order[-1]), int(order[-2])]
+    rest = [i for i in range(len(sizes)) if i not in pick]
+    rng = np.random.default_rng(seed)
+    pick += [int(i) for i in rng.choice(rest, size=k_random, replace=False)]
+    return pick
+
+
+def test_batch_64_engine_equals_lone_solves_and_oracle(gpu_lib, oracle32, oracle64, monkeypatch):
+    import torch
+    from disptools_amd.workloads import batch_uneven
+    # one column tile on both sides: a lone solve picks its tile per shape,
+    # an engine lane uses one
+    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
+    cases = batch_uneven(64, workers=8)
+    print("batch of", len(cases), "volumes,", sum(c[0].size for c in cases), "voxels", flush=True)
+    devs = [_DevVolume(T, M) for T, M, _ in cases]
+    torch.cuda.synchronize()
+    with gpu_lib.Engine(tz=-1, max_slots=64) as eng:
+        tickets = {}
+        for i, (T, M, sp) in enumerate(cases):
+            nz, ny, nx = T.shape
+            t = eng.submit("gradient", nx, ny, nz, sp, devs[i].T.data_ptr(), devs[i].M.data_ptr(),
+                           devs[i].F.data_ptr(), it_max=2000)
+            assert t is not None
+            tickets[t] = i
+        eng.drain()
+        res = {tickets[t]: (s, info) for t, s, info in eng.poll()}
+    torch.cuda.synchronize()
+    assert sorted(res) == list(range(len(cases)))
+    fields = [d.F.cpu().numpy() for d in devs]
+    print("engine done; lone solves", flush=True)
+    for i, (T, M, sp) in enumerate(cases):
+        d = devs[i]
+        d.F.zero_()
+        nz, ny, nx = T.shape
+        info = gpu_lib.solve_device("gradient", nx, ny, nz, sp, d.T.data_ptr(), d.M.data_ptr(), d.F.data_ptr(),
+                                    it_max=2000)[0]
+        torch.cuda.synchronize()
+        assert res[i][0] == 0
+        for k in SAME_KEYS:
+            assert res[i][1][k] == info[k], (i, k, res[i][1][k], info[k])
+        np.testing.assert_array_equal(fields[i], d.F.cpu().numpy())
+    pick = _oracle_subset([c[0].size for c in cases])
+    with _oracle_threads(oracle32, oracle64):
+        for i in pick:
+            T, M, sp = cases[i]
+            _check_against_fp64(oracle32, oracle64, "gradient", T, M, sp, 2000, fields[i], res[i][1], "batch[%d]" % i)
+
+
+def test_serving_stream_30_requests(gpu_lib, oracle32, oracle64, monkeypatch):
+    from disptools_amd.serving import SolveRequest, SolveServer
+    from disptools_amd.workloads import serving_stream, volume_case
+    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
+    monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", "1")
+    stream = serving_stream(30, 80.0)
+    reqs = []
+    for k, r in enumerate(stream):
+        side = r["side"]
+        T, M = volume_case((side, side, side), r["seed"])
+        reqs.append(SolveRequest(T, M, (1.0, 1.0, 1.0), r["algorithm"], {"it_max": 2000}, request_id=str(k)))
+    print("30 requests, sides", [r["side"] for r in stream], flush=True)
+    with SolveServer(tz=-1) as srv:
+        out = {r.request_id: r for r in srv.run(reqs)}
+    assert sorted(out, key=int) == [r.request_id for r in reqs]
+    for r in reqs:
+        res = out[r.request_id]
+        assert res.status == 0, (r.request_id, res.reason_text)
+        f, info, _ = gpu_lib.solve_host(r.algorithm, r.target, r.mask, r.spacing, **r.params)
+        assert (res.reason, res.iterations, res.evaluations, res.rejections) == \
+            (info["reason"], info["iterations"], info["evaluations"], info["rejections"]), r.request_id
+        assert res.final_max_error == info["final_max_error"]
+        np.testing.assert_array_equal(res.field, f)
+    pick = _oracle_subset([r.target.size for r in reqs])
+    with _oracle_threads(oracle32, oracle64):
+        for k in pick:
+            r = reqs[k]
+            res = out[r.request_id]
+            info = dict(reason=res.reason, iterations=res.iterations)
+            _check_against_fp64(oracle32, oracle64, r.algorithm, r.target, r.mask, r.spacing, 2000, res.field,
+                                info, "request %s" % r.request_id)
+
+
+def test_default_call_runs_on_the_current_device(gpu_lib, oracle32, oracle64):
+    """displacement(J, mask=M) with every argument at the reference default
+    (gpu_id = -1, levels = 1, it_max = 50000, zeta = 10): the reference's CPU
+    branch, here generate_displacement_gradient on the current HIP device.
+    The result holds the parity bar against the FP64 oracle solving the same
+    padded volume."""
+    from disptools_amd import displacements
+    from disptools_amd.workloads import sphere_case
+    T, M, sp = sphere_case(24, radius=7.0)
+    out = displacements.displacement(T, mask=M)
+    assert out.shape == T.shape + (3,)
+    # the solve the call makes: one voxel of padding, J edge-replicated, mask 0
+    Tp = np.pad(T, 1, mode="edge")
+    Mp = np.pad(M, 1, mode="constant")
+    fo, io = oracle64.solve("gradient", Tp.astype(np.float64), Mp, sp, it_max=50000, zeta=10.0)
+    fo = np.moveaxis(fo.astype(np.float32), 0, -1)[1:-1, 1:-1, 1:-1]
+    field = np.ascontiguousarray(np.moveaxis(out, -1, 0))
+    mg = _masked_max(oracle32, field, T, M, sp)
+    mo = _masked_max(oracle32, np.ascontiguousarray(np.moveaxis(fo, -1, 0)), T, M, sp)
+    print("default call: max err %.7g, fp64 oracle %.7g (reason %d)" % (mg, mo, io["reason"]))
+    assert abs(mg - mo) <= MAX_ERR_TOL
+    # the extension entry point with gpu_id = -1 gives the same field
+    from disptools_amd import load_extension
+    ext = load_extension()
+    field2 = np.zeros((3,) + Tp.shape, np.float32)
+    p = gpu_lib.DEFAULT_PARAMS
+    ext.displacement(sp, Tp, Mp, p["epsilon"], p["tolerance"], p["eta"], p["eta_max"], p["alpha"], p["beta"],
+                     p["gamma"], p["delta"], 10.0, p["theta"], p["iota"], False, 50000, field2, "gradient", -1)
+    np.testing.assert_array_equal(np.moveaxis(field2, 0, -1)[1:-1, 1:-1, 1:-1], out)

@@ -51,3 +51,28 @@ def test_server_solves_matching_requests(gpu_lib):
     assert (res.reason, res.iterations) == (im["reason"], im["iterations"])
     assert res.final_max_error == im["final_max_error"]
     assert res.evaluations == im["cost_passes"] + im["gradient_passes"]
+
+
+def test_server_answers_a_refused_request_and_keeps_serving(gpu_lib):
+    """A request the engine refuses (here: larger than the engine's pool)
+    comes back as a failed response; the valid request beside it is solved."""
+    from conftest import sphere
+    from disptools_amd.serving import SolveRequest, SolveServer, check_request
+    T1, M1 = sphere(16, 5)
+    T2, M2 = sphere(40, 12)
+    small = SolveRequest(T1, M1, (1.0, 1.0, 1.0), "gradient", {"it_max": 100}, request_id="small")
+    big = SolveRequest(T2, M2, (1.0, 1.0, 1.0), "gradient", {"it_max": 100}, request_id="big")
+    with pytest.raises(ValueError):
+        check_request((2, 16, 16), "gradient", {})
+    probe = gpu_lib.Engine(hbm_budget=1 << 26)
+    fp_small = probe.footprint("gradient", 16, 16, 16, host_io=True)
+    fp_big = probe.footprint("gradient", 40, 40, 40, host_io=True)
+    probe.close()
+    assert fp_small < fp_big
+    with SolveServer(hbm_budget=fp_small + (fp_big - fp_small) // 2) as srv:
+        out = {r.request_id: r for r in srv.run([big, small])}
+    assert set(out) == {"big", "small"}
+    assert out["big"].status == -1 and "hbm_budget" in out["big"].reason_text
+    assert out["small"].status == 0 and out["small"].reason != 0
+    f, info, _ = gpu_lib.solve_host("gradient", T1, M1, it_max=100)
+    assert out["small"].reason == info["reason"]
