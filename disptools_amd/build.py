@@ -37,8 +37,6 @@ HIP_FLAGS = [
 # standalone jacobian keeps every multiply and add separately rounded (it is
 # bit-exact with the reference); the solver pass is FP32 with FMA contraction.
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
-               ("dt_grad_pass.hip", "-ffp-contract=fast"),
-               ("dt_split_pass.hip", "-ffp-contract=fast"),
                ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_matching.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_pyramid.hip", "-ffp-contract=off"),

@@ -1,5 +1,6 @@
-// dt_device.hpp -- device helpers shared by the pass kernels: block reductions,
-// the last-block arrival protocol, the solver controllers and tile decoding.
+// dt_device.hpp -- device helpers shared by the pass kernels: exact pass sums,
+// the solver controllers, the controller step at the start of a pass and tile
+// decoding.
 #pragma once
 
 #include <math.h>
@@ -45,14 +46,38 @@ __device__ __forceinline__ T *uni(T *p)
 }
 
 // ---------------------------------------------------------------------------
-// Reductions
+// Exact sums (the Acc layout: 128-bit fixed point, quantum 2^-64, 32-bit limbs)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ double wave_sum(double v)
+typedef unsigned __int128 u128;
+
+// Adds the term d >= 0 to a fixed-point sum: floor(d * 2^64).  Terms at or
+// above 2^32 (or infinite) set the `huge` flag bit and NaN the `nan` bit
+// instead: the sum is then reported as +inf or NaN.  The three 32-bit limbs
+// come from truncating conversions: floor(d), then the fraction scaled by
+// 2^32 twice; every subtraction and scaling is exact (d has 53 significant
+// bits, the fraction of an exact integer split is representable), so the
+// limbs are the exact binary digits of d down to 2^-64.
+__device__ __forceinline__ void fix_add(u128 &acc, double d, unsigned &flags, unsigned huge, unsigned nan)
 {
+    const bool is_nan = d != d;
+    const bool is_huge = d >= 0x1p32;
+    const double x = (is_nan || is_huge) ? 0.0 : d;
+    const unsigned l2 = __double2uint_rz(x);
+    double r = (x - (double)l2) * 0x1p32;
+    const unsigned l1 = __double2uint_rz(r);
+    r = (r - (double)l1) * 0x1p32;
+    const unsigned l0 = __double2uint_rz(r);
+    acc += ((u128)l2 << 64) | ((u128)l1 << 32) | (u128)l0;
+    flags |= (is_huge ? huge : 0u) | (is_nan ? nan : 0u);
+}
+
+__device__ __forceinline__ u128 shfl_xor_u128(u128 v, int o)
+{
+    unsigned w[4];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int k = 0; k < 4; ++k) w[k] = (unsigned)__shfl_xor((int)(unsigned)(v >> (32 * k)), o, 64);
+    return (u128)w[0] | ((u128)w[1] << 32) | ((u128)w[2] << 64) | ((u128)w[3] << 96);
 }
 
 __device__ __forceinline__ float wave_max(float v)
@@ -62,98 +87,126 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// Block reduction of (sum a, max b, sum c) to thread 0, fixed order.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// LDS scratch of the end-of-pass reduction: one entry per wave.
+struct SumsWave {
+    u128 e, g;
+    float mx;
+    unsigned flags;
+};
+
+// Adds a block's sums into its copy of accumulator slot `slot`: wave sums by
+// shuffles, the waves' sums in LDS, then one thread's atomics (relaxed,
+// agent scope: the pass that reads them starts after this kernel has ended).
 template <int NT>
-__device__ __forceinline__ void block_reduce3(double &a, float &b, double &c, double *red)
+__device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx, unsigned flags, SumsWave *red)
 {
     constexpr int NW = NT / 64;
-    a = wave_sum(a);
-    b = wave_max(b);
-    c = wave_sum(c);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        e += shfl_xor_u128(e, o);
+        g += shfl_xor_u128(g, o);
+        flags |= (unsigned)__shfl_xor((int)flags, o, 64);
+    }
+    mx = wave_max(mx);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
     if (lane == 0) {
-        red[3 * w + 0] = a;
-        red[3 * w + 1] = (double)b;
-        red[3 * w + 2] = c;
+        red[w].e = e;
+        red[w].g = g;
+        red[w].mx = mx;
+        red[w].flags = flags;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double sa = 0.0, sc = 0.0;
-        float mb = 0.0f;
+        u128 se = 0, sg = 0;
+        float m = 0.0f;
+        unsigned f = 0;
         for (int i = 0; i < NW; ++i) {
-            sa += red[3 * i + 0];
-            mb = fmaxf(mb, (float)red[3 * i + 1]);
-            sc += red[3 * i + 2];
+            se += red[i].e;
+            sg += red[i].g;
+            m = fmaxf(m, red[i].mx);
+            f |= red[i].flags;
         }
-        a = sa; b = mb; c = sc;
+        Acc *a = I.acc + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long le = (unsigned long long)(unsigned)(se >> (32 * k));
+            const unsigned long long lg = (unsigned long long)(unsigned)(sg >> (32 * k));
+            if (le) __hip_atomic_fetch_add(&a->e[k], le, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lg) __hip_atomic_fetch_add(&a->g[k], lg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (m > 0.0f) __hip_atomic_fetch_max(&a->mx, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (f) __hip_atomic_fetch_or(&a->flags, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-__device__ __forceinline__ void store_partial(double *p, double v)
+// 128-bit fixed-point value (quantum 2^-64) as a double; deterministic.
+__device__ __forceinline__ double fix_to_double(u128 v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi = (unsigned long long)(v >> 64), lo = (unsigned long long)v;
+    return (double)hi + (double)lo * 0x1p-64;
 }
 
-__device__ __forceinline__ double load_partial(const double *p)
-{
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// The sums of one accumulator slot, from its ACC_COPIES copies as 64-bit
+// words (the layout of Acc: e[4], g[4], mx | flags << 32).
+struct PassSums {
+    double E, G;
+    float m;
+};
 
-// Publishes this block's partials; returns true in every thread of the block
-// that arrived last for its instance.
-template <int NT>
-__device__ bool arrive(const Inst &I, int tile, double E, float m, double G, int *s_flag)
+constexpr int ACC_WORDS = (int)(sizeof(Acc) / 8);
+static_assert(ACC_WORDS == 9, "Acc is nine 64-bit words");
+
+__device__ inline PassSums sums_of_slot(const unsigned long long *w)
 {
-    if (threadIdx.x == 0) {
-        double *p = I.partials + 4 * (long long)tile;
-        store_partial(p + 0, E);
-        store_partial(p + 1, (double)m);
-        store_partial(p + 2, G);
-        const unsigned prev = __hip_atomic_fetch_add(I.counter, 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        *s_flag = (prev + 1 == (unsigned)(I.ntx * I.nty * I.ntz)) ? 1 : 0;
+    u128 e = 0, g = 0;
+    unsigned mxb = 0, flags = 0;
+    for (int k = 0; k < ACC_COPIES; ++k) {
+        const unsigned long long *a = w + k * ACC_WORDS;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            e += (u128)a[l] << (32 * l);
+            g += (u128)a[4 + l] << (32 * l);
+        }
+        const unsigned m = (unsigned)a[8];
+        mxb = m > mxb ? m : mxb;
+        flags |= (unsigned)(a[8] >> 32);
     }
-    __syncthreads();
-    return *s_flag != 0;
-}
-
-// Deterministic fold of an instance's partials by the last block.
-template <int NT>
-__device__ void fold_partials(const Inst &I, double &E, float &m, double &G, double *red)
-{
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    const int ntiles = I.ntx * I.nty * I.ntz;
-    double a = 0.0, c = 0.0;
-    float b = 0.0f;
-    for (int t = threadIdx.x; t < ntiles; t += NT) {
-        const double *p = I.partials + 4 * (long long)t;
-        a += load_partial(p + 0);
-        b = fmaxf(b, (float)load_partial(p + 1));
-        c += load_partial(p + 2);
-    }
-    block_reduce3<NT>(a, b, c, red);
-    E = a; m = b; G = c;
+    PassSums s;
+    s.E = (flags & ACC_E_NAN) ? (double)NAN : ((flags & ACC_E_HUGE) ? (double)INFINITY : fix_to_double(e));
+    s.G = (flags & ACC_G_NAN) ? (double)NAN : ((flags & ACC_G_HUGE) ? (double)INFINITY : fix_to_double(g));
+    s.m = __uint_as_float(mxb);
+    return s;
 }
 
 // ---------------------------------------------------------------------------
-// Controllers (run by one thread after the instance's partials are folded)
+// Controllers (run at the start of the next pass, on a copy of the state)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, double it, double E, double m, double eta)
+// A log line (it, E, max error, eta); only the instance's first tile writes
+// the trace (`leader`), every block counts it.
+__device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, bool leader, double it, double E, double m,
+                                         double eta)
 {
-    if (I.trace && C->trace_len < I.trace_cap) {
+    if (leader && I.trace && C->trace_len < I.trace_cap) {
         double *t = I.trace + 4 * C->trace_len;
         t[0] = it; t[1] = E; t[2] = m; t[3] = eta;
     }
     C->trace_len++;
 }
 
-// Completion word of a solve in host-coherent memory: (active passes << 1) |
-// done, stored after every controller step, so the host follows the solve
-// without a copy or an event between pass launches.  Once the solve is done
-// the control block itself is published first (when the instance has a host
-// copy), ordered before the word by a system-scope release.
+// Completion word of a solve in host-coherent memory: (evaluated passes << 1)
+// | done, stored by the instance's first tile after every controller step, so
+// the host follows the solve without a copy or an event between pass
+// launches.  Once the solve is done the control block itself is published
+// first (when the instance has a host copy), ordered before the word by a
+// system-scope release.
 __device__ __forceinline__ void publish_status(const Inst &I, const Ctrl *C)
 {
     if (!I.status) return;
@@ -213,7 +266,7 @@ __device__ __forceinline__ void start_interp(Ctrl *C, const dt_params &p, bool s
 // Gradient solver control flow (src/displacement_field_gradient.c:250-421).
 // E, m, G are the energy, masked max error and squared gradient norm of the
 // trial the pass evaluated.
-static __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m, double G)
+static __device__ void grad_controller(const Inst &I, Ctrl *C, bool leader, double E, float m, double G)
 {
     const dt_params &p = I.p;
     C->evaluations++;
@@ -226,7 +279,7 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m
         C->E0 = E; C->m0 = m;
         C->E_last = E;
         C->G_base = G;
-        log_line(I, C, 0.0, E, m, C->eta);
+        log_line(I, C, leader, 0.0, E, m, C->eta);
         if (p.it_max <= 0) { finish(C, DT_STOP_MAX_ITERATIONS); return; }
         C->cand_buf = C->out_next;
         C->eta_c = C->eta_next;
@@ -265,7 +318,7 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m
             start_interp(C, p, true);
             return;
         }
-        log_line(I, C, (double)C->it, E, m, C->eta);
+        log_line(I, C, leader, (double)C->it, E, m, C->eta);
         if (!isnormal((float)E)) { finish(C, DT_STOP_ERROR_EXPLODED); return; }
         if (C->eta < p.iota) { finish(C, DT_STOP_ETA_SMALL); return; }
         if (1.0 - E / C->E_last < (double)p.theta) { finish(C, DT_STOP_NO_IMPROVEMENT); return; }
@@ -297,10 +350,9 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, double E, float m
 }
 
 // Greedy solver control flow (src/displacement_field_greedy.c:149-238).
-// `step_io` is the field volumes a pass with the step moves (step kernel +
-// evaluation kernel: 3; fused pass: 2); the evaluation of the initial guess
-// reads one.
-static __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float m, int step_io)
+// `step_io` is the field volumes a pass with the step moves (2: the base read,
+// the trial written); the evaluation of the initial guess reads one.
+static __device__ void greedy_controller(const Inst &I, Ctrl *C, bool leader, double E, float m, int step_io)
 {
     const dt_params &p = I.p;
     C->evaluations++;
@@ -320,7 +372,7 @@ static __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float
         return;
     }
     // the trial is buf[g_new]; its error map is err[e_written]
-    log_line(I, C, (double)C->it, E, m, C->eta);
+    log_line(I, C, leader, (double)C->it, E, m, C->eta);
     C->e_cur = e_written;                 // the next step reads the newest map (:165-181)
     C->e_new = 1 - e_written;
     if (!isnormal((float)E)) { finish(C, DT_STOP_ERROR_EXPLODED); return; }
@@ -341,6 +393,81 @@ static __device__ void greedy_controller(const Inst &I, Ctrl *C, double E, float
     C->base = nb;
     C->it++;
     if (C->it > p.it_max) { finish(C, DT_STOP_MAX_ITERATIONS); return; }
+}
+
+// ---------------------------------------------------------------------------
+// The start of a pass
+// ---------------------------------------------------------------------------
+
+// LDS of pass_begin: every accumulator slot of the instance, as words.
+constexpr int BEGIN_ACC_WORDS = ACC_SLOTS * ACC_COPIES * ACC_WORDS;
+
+// Run by wave 0 of every block before the block's work: copies the control
+// block of this pass's parity into `c` (the block's LDS copy) and, when it
+// records an evaluated trial, advances the controller (ALG: DT_GRADIENT or
+// DT_GREEDY) by the sums of that pass.  Every block computes the same state
+// from the same inputs; the instance's first tile (`leader`) stores it into
+// the other copy for the next pass, clears the accumulator slot the next pass
+// adds into and publishes the completion word.  On return `c` is the state
+// this pass works from (active == 0: the solve has ended, the pass does
+// nothing); the caller synchronises the block before reading it.
+//
+// The lanes load the control words and every slot's sums at once (which slot
+// holds the previous pass's sums is in the control block itself), so the
+// whole step costs one memory round trip; lane 0 then runs the controller on
+// the LDS copy (in registers the control block would raise the register
+// count of the kernel).
+template <int ALG>
+__device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c, unsigned long long *acc_w)
+{
+    constexpr int W = (int)(sizeof(Ctrl) / 8);
+    constexpr int AR = (BEGIN_ACC_WORDS + 63) / 64;
+    static_assert(W <= 64, "one control word per lane");
+    const int lane = threadIdx.x & 63;
+    const unsigned long long *src = (const unsigned long long *)(I.ctrl + parity);
+    const unsigned long long *asrc = (const unsigned long long *)I.acc;
+    unsigned long long *dst = (unsigned long long *)c;
+    unsigned long long cw = 0, aw[AR];
+    if (lane < W) cw = src[lane];
+#pragma unroll
+    for (int k = 0; k < AR; ++k) {
+        const int i = lane + 64 * k;
+        aw[k] = i < BEGIN_ACC_WORDS ? asrc[i] : 0ull;
+    }
+    if (lane < W) dst[lane] = cw;
+#pragma unroll
+    for (int k = 0; k < AR; ++k) {
+        const int i = lane + 64 * k;
+        if (i < BEGIN_ACC_WORDS) acc_w[i] = aw[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) {
+        if (c->active && c->evaluated) {
+            const PassSums s = sums_of_slot(acc_w + c->acc_cur * ACC_COPIES * ACC_WORDS);
+            if (ALG == DT_GRADIENT) grad_controller(I, c, leader, s.E, s.m, s.G);
+            else greedy_controller(I, c, leader, s.E, s.m, 2);
+            const int prev = c->acc_prev;
+            c->acc_prev = c->acc_cur;
+            c->acc_cur = c->acc_zero;
+            c->acc_zero = prev;
+        }
+        c->evaluated = c->active;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (leader) {
+        // an ended solve stays ended at both parities
+        unsigned long long *next = (unsigned long long *)(I.ctrl + (parity ^ 1));
+        if (lane < W) next[lane] = dst[lane];
+        if (c->active) {
+            unsigned long long *z = (unsigned long long *)(I.acc + (long long)c->acc_zero * ACC_COPIES);
+            for (int i = lane; i < ACC_COPIES * ACC_WORDS; i += 64) z[i] = 0ull;
+        }
+        if (lane == 0) publish_status(I, c);
+    }
 }
 
 // ---------------------------------------------------------------------------

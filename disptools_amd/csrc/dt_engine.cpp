@@ -6,10 +6,11 @@
 // algorithm into lanes; a lane has its own stream, tile table and chunk
 // pipeline, so the passes of one algorithm never wait for the other's.  A
 // pass of the gradient lane is one launch of the fused gradient kernel over
-// the tile table of all its instances, a pass of the greedy lane the greedy
-// step and evaluation over all of its; each instance's last block runs that
-// instance's controller, so step lengths, line searches and stopping tests
-// stay per instance and never wait for the host.  Per lane the host enqueues
+// the tile table of all its instances, a pass of the greedy lane the fused
+// greedy kernel over all of its; each block starts by advancing its
+// instance's controller on the previous pass's sums (the blocks of an
+// instance all reach the same state), so step lengths, line searches and
+// stopping tests stay per instance and never wait for the host.  Per lane the host enqueues
 // chunks of passes (two in flight).  Each controller stores its instance's
 // completion word (and, once done, its control block) in host-coherent
 // memory, so only pass kernels enter a lane's stream: the host follows the
@@ -18,8 +19,8 @@
 // rebuilt whenever the lane's set of instances changes.  A chunk can be
 // replayed from a captured hipGraph.
 //
-// An instance's result does not depend on what else is in the batch: its
-// tiles, the order of its partial sums and its controller are its own.
+// An instance's result does not depend on what else is in the batch, nor on
+// its tiling: its sums are exact (fixed point) and its controller is its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -171,14 +172,15 @@ struct dt_engine {
     Pool pool;
 
     Inst *d_insts = nullptr;
-    Ctrl *d_ctrl = nullptr;
-    unsigned int *d_counter = nullptr;
+    Ctrl *d_ctrl = nullptr;            // two copies per slot (pass parity)
+    Acc *d_acc = nullptr;              // ACC_SLOTS * ACC_COPIES per slot
+    unsigned int *d_counter = nullptr; // matching instances' arrival counters
     Inst *h_insts = nullptr;     // pinned mirror, one per slot
     MInst *d_minsts = nullptr;   // matching instances, slot-indexed
     MCtrl *d_mctrl = nullptr;
     MInst *h_minsts = nullptr;   // pinned mirror
     MCtrl *h_mctrl0 = nullptr;   // pinned initial control blocks
-    Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks
+    Ctrl *h_ctrl0 = nullptr;     // pinned initial control blocks (two copies per slot)
     int *h_zero = nullptr;       // pinned 0 (deactivates a stuck instance)
     // host-coherent, per slot: completion words, then final control blocks
     char *coherent = nullptr;
@@ -214,8 +216,10 @@ int engine_tz(const dt_engine *e, int nz)
 {
     if (e->tz_policy > 0) return std::min(e->tz_policy, nz);
     if (e->tz_policy < 0) return 0;   // make_plan: as a lone solve
-    const int chunks = (nz + 15) / 16;
-    return (nz + chunks - 1) / chunks;
+    // balanced chunks of at most 16 planes, in whole groups of four
+    const int groups = (nz + 3) / 4;
+    const int chunks = (groups + 3) / 4;
+    return (groups + chunks - 1) / chunks * 4;
 }
 
 Plan engine_plan(const dt_engine *e, int algorithm, int nx, int ny, int nz)
@@ -301,11 +305,13 @@ int rebuild_tables(dt_engine *e, Lane &L, int stage)
 
 int launch_passes(dt_engine *e, const Lane &L, hipStream_t s)
 {
+    // chunks have an even length, so every instance's passes alternate the
+    // parity from its first one (instances join between chunks)
     for (int i = 0; i < e->chunk; ++i) {
         if (L.kind == DT_GRADIENT) {
-            DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, L.d_tiles, L.ntiles, s));
+            DT_HIP(launch_grad_pass(e->precision, e->variant, e->d_insts, L.d_tiles, L.ntiles, i & 1, s));
         } else if (L.kind == DT_GREEDY) {
-            DT_HIP(launch_greedy_pass(e->precision, e->greedy_variant, e->d_insts, L.d_tiles, L.ntiles, s));
+            DT_HIP(launch_greedy_pass(e->precision, e->greedy_variant, e->d_insts, L.d_tiles, L.ntiles, i & 1, s));
         } else {
             DT_HIP(launch_match_batch(e->d_minsts, L.d_tiles, L.ntiles, s));
         }
@@ -504,10 +510,13 @@ int check_stuck(dt_engine *e, Lane &L, const Chunk &c)
             continue;
         }
         if (sl.launched <= max_passes(sl.params)) continue;
-        DT_HIP(hipMemcpyAsync(&e->d_ctrl[n.slot].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
+        DT_HIP(hipMemcpyAsync(&e->d_ctrl[2 * n.slot].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice, L.st));
+        DT_HIP(hipMemcpyAsync(&e->d_ctrl[2 * n.slot + 1].active, e->h_zero, sizeof(int), hipMemcpyHostToDevice,
+                              L.st));
         DT_HIP(hipStreamSynchronize(L.st));
-        Ctrl f;
-        DT_HIP(hipMemcpy(&f, &e->d_ctrl[n.slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+        Ctrl fc[2];
+        DT_HIP(hipMemcpy(fc, &e->d_ctrl[2 * n.slot], sizeof fc, hipMemcpyDeviceToHost));
+        Ctrl f = latest_ctrl(fc);
         const int failed = f.phase != PH_DONE;
         if (failed) f.result_buf = f.base;
         if (retire(e, n.slot, f, failed)) return -1;
@@ -554,8 +563,9 @@ int check_lanes(dt_engine *e)
                         ((unsigned long long)(m.cost_passes + m.grad_passes) << 1) | (mdone ? 1ull : 0ull);
                     continue;
                 }
-                Ctrl f;
-                DT_HIP(hipMemcpy(&f, &e->d_ctrl[n.slot], sizeof(Ctrl), hipMemcpyDeviceToHost));
+                Ctrl fc[2];
+                DT_HIP(hipMemcpy(fc, &e->d_ctrl[2 * n.slot], sizeof fc, hipMemcpyDeviceToHost));
+                const Ctrl &f = latest_ctrl(fc);
                 const bool done = f.phase == PH_DONE;
                 if (done) std::memcpy(final_of(e, n.slot), &f, sizeof(Ctrl));
                 *status_of(e, n.slot) = ((unsigned long long)f.evaluations << 1) | (done ? 1ull : 0ull);
@@ -652,6 +662,7 @@ void destroy(dt_engine *e)
     if (e->coherent) (void)hipHostFree(e->coherent);
     if (e->d_insts) (void)hipFree(e->d_insts);
     if (e->d_ctrl) (void)hipFree(e->d_ctrl);
+    if (e->d_acc) (void)hipFree(e->d_acc);
     if (e->d_counter) (void)hipFree(e->d_counter);
     if (e->pool_mem) (void)hipFree(e->pool_mem);
     if (e->io) (void)hipStreamDestroy(e->io);
@@ -669,7 +680,8 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     // short chunks: no host round trip separates them, and an admitted
     // instance joins at the next chunk (64 volumes U{48..160}^3: 2 passes per
     // chunk: 1 % faster than 4, 3.5 % than 8; serving percentiles unchanged)
-    e->chunk = (cfg && cfg->chunk > 0) ? std::min(cfg->chunk, 64) : 2;
+    // (an even length: see launch_passes)
+    e->chunk = (cfg && cfg->chunk > 0) ? std::min((cfg->chunk + 1) / 2 * 2, 64) : 2;
     e->use_graphs = (cfg && cfg->use_graphs) ? 1 : 0;
     e->status_words = status_words_from_env();
     e->tz_policy = cfg ? cfg->tz : 0;
@@ -706,12 +718,14 @@ int create(const dt_engine_config *cfg, void *stream, dt_engine *e)
     }
     e->pool.init(budget);
     DT_HIP(hipMalloc(&e->d_insts, (size_t)S * sizeof(Inst)));
-    DT_HIP(hipMalloc(&e->d_ctrl, (size_t)S * sizeof(Ctrl)));
+    DT_HIP(hipMalloc(&e->d_ctrl, (size_t)S * 2 * sizeof(Ctrl)));
+    DT_HIP(hipMalloc(&e->d_acc, (size_t)S * ACC_SLOTS * ACC_COPIES * sizeof(Acc)));
     DT_HIP(hipMalloc(&e->d_counter, (size_t)S * sizeof(unsigned int)));
-    DT_HIP(hipMemset(e->d_ctrl, 0, (size_t)S * sizeof(Ctrl)));
+    DT_HIP(hipMemset(e->d_ctrl, 0, (size_t)S * 2 * sizeof(Ctrl)));
+    DT_HIP(hipMemset(e->d_acc, 0, (size_t)S * ACC_SLOTS * ACC_COPIES * sizeof(Acc)));
     DT_HIP(hipMemset(e->d_counter, 0, (size_t)S * sizeof(unsigned int)));
     DT_HIP(hipHostMalloc((void **)&e->h_insts, (size_t)S * sizeof(Inst), hipHostMallocDefault));
-    DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * sizeof(Ctrl), hipHostMallocDefault));
+    DT_HIP(hipHostMalloc((void **)&e->h_ctrl0, (size_t)S * 2 * sizeof(Ctrl), hipHostMallocDefault));
     DT_HIP(hipHostMalloc((void **)&e->h_zero, sizeof(int), hipHostMallocDefault));
     *e->h_zero = 0;
     DT_HIP(hipMalloc(&e->d_minsts, (size_t)S * sizeof(MInst)));
@@ -878,18 +892,19 @@ long long submit(dt_engine *e, int algorithm, size_t nx, size_t ny, size_t nz, f
     sl.out_field = field;
     if (!rc) {
         *status_of(e, s) = 0;
+        Acc *acc = e->d_acc + (size_t)s * ACC_SLOTS * ACC_COPIES;
         e->h_insts[s] = make_inst(P, dx, dy, dz, *params, d_target, d_mask, d_field, cv, nullptr, 0,
-                                  &e->d_ctrl[s], &e->d_counter[s]);
+                                  &e->d_ctrl[2 * s], acc);
         if (e->status_words) {
             e->h_insts[s].status = (unsigned long long *)e->coherent_dev + s;
             e->h_insts[s].final_ctrl = (Ctrl *)(e->coherent_dev + e->final_off) + s;
         }
-        e->h_ctrl0[s] = initial_ctrl(algorithm, *params);
+        e->h_ctrl0[2 * s] = e->h_ctrl0[2 * s + 1] = initial_ctrl(algorithm, *params);
         if (hipMemcpyAsync(&e->d_insts[s], &e->h_insts[s], sizeof(Inst), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||
-            hipMemcpyAsync(&e->d_ctrl[s], &e->h_ctrl0[s], sizeof(Ctrl), hipMemcpyHostToDevice, L.st) !=
+            hipMemcpyAsync(&e->d_ctrl[2 * s], &e->h_ctrl0[2 * s], 2 * sizeof(Ctrl), hipMemcpyHostToDevice, L.st) !=
                 hipSuccess ||
-            hipMemsetAsync(&e->d_counter[s], 0, sizeof(unsigned int), L.st) != hipSuccess) {
+            hipMemsetAsync(acc, 0, ACC_SLOTS * ACC_COPIES * sizeof(Acc), L.st) != hipSuccess) {
             set_error(__LINE__, __FILE__, __func__, "cannot upload the instance record");
             rc = -1;
         }

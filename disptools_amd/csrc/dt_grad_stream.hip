@@ -1,10 +1,12 @@
 // dt_grad_stream.hip -- the fused gradient pass, single-barrier streaming form.
 //
-// Same computation as k_grad_pass (Jacobian, voxel error, penalised weight,
-// weighted cofactors on plane zz; adjoint-stencil gradient and speculative
-// candidate on plane zz-1; FP64 partial sums; controller in the last block),
-// src/displacement_field_gradient.c:26-175 and src/jacobian.c:70-242, with a
-// work layout balanced over the four SIMDs of a CU:
+// One launch per trial: Jacobian, voxel error, penalised weight and weighted
+// cofactors on plane zz; adjoint-stencil gradient and speculative candidate
+// on plane zz-1; exact sums of e^2 and |g|^2 (src/displacement_field_gradient.c:26-175,
+// src/jacobian.c:70-242).  The controller step that chose the trial runs at
+// the start of the launch, in every block, on the previous launch's sums
+// (pass_begin); nothing synchronises the blocks within a launch.  Work layout,
+// balanced over the four SIMDs of a CU:
 //
 //  * every thread has one primary point: the first TX*TY threads own the
 //    tile's columns, the next 2(TX+TY) threads (in waves of their own) the
@@ -51,8 +53,10 @@ struct StreamSmem {
     float xs[3][3][G::PL];            // ring of three trial planes, halo 3
     float wx[2][3][TY][TX + 2];       // W_{a,x} by parity, x-halo 1
     float wy[2][3][TY + 2][TX];       // W_{a,y} by parity, y-halo 1
-    double red[3 * (G::NT / 64)];
-    int flag;
+    u128 accE[G::NOWN], accG[G::NOWN];   // exact sums of the tile threads (kept here, not in registers)
+    SumsWave red[G::NT / 64];
+    Ctrl ctrl;                        // this pass's control state (pass_begin)
+    unsigned long long acc_w[BEGIN_ACC_WORDS];   // the instance's accumulators (pass_begin)
 };
 
 // (xx, yy) in the halo-3 plane of the r-th point of rings 2-3 and ring-1 corners.
@@ -83,7 +87,7 @@ __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 template <typename CT>
 __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, int zz, int gx, int gy,
                                                const float (&sm)[3][3][4], const float (&v0)[3], float T,
-                                               bool in_mask, bool count, double &Esum, float &mx, CT (&W)[9])
+                                               bool in_mask, bool count, double &Egrp, float &mx, CT (&W)[9])
 {
     // planes away from the z faces: the column's own coefficients (no
     // wave-wide branch on boundary lanes); the four face planes: region map
@@ -92,7 +96,7 @@ __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoe
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     if (count) {
-        Esum += (double)e * (double)e;
+        Egrp += (double)e * (double)e;
         if (in_mask) mx = fmaxf(mx, fabsf(e));
     }
     const CT cw = (CT)penalised_weight(e, J, P);
@@ -116,10 +120,15 @@ __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoe
     W[8] = cw * (d2[0][0] * d2[1][1] - d2[0][1] * d2[1][0]);
 }
 
+// The sums are exact (fixed point, dt_device.hpp); their terms are the double
+// sums of e^2 and |g|^2 over groups of four planes of one column, the groups
+// starting at z = 0 mod 4 (z-chunks start on group boundaries), so the sums
+// do not depend on the tiling.
 template <typename CT, int TX, int TY, bool INTERP>
 __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const Inst &I, const Ctrl *C,
-                                                 const TileCoord tc, double &Esum, float &mx, double &Gsum)
+                                                 const TileCoord tc, unsigned &flags, float &mx)
 {
+    double Egrp = 0.0, Ggrp = 0.0;   // sums of the current group of planes
     using G = StreamGeom<TX, TY>;
     constexpr int NT = G::NT, NOWN = G::NOWN, HX = G::HX, PL = G::PL, NRING = G::NRING;
     constexpr int NOTHER = G::NOTHER, SX = G::SX, SLOTS = G::SLOTS;
@@ -161,6 +170,17 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         what = 3;
     }
     const int gx = tc.x0 + lx, gy = tc.y0 + ly;
+    if (own) { S.accE[tid] = 0; S.accG[tid] = 0; }
+    // a finished group of planes into the thread's exact sums (tile threads
+    // only; `own` is uniform over a wave)
+    auto flush = [&](u128 *acc, double &grp, unsigned huge, unsigned nan) {
+        if (own) {
+            u128 a = acc[tid];
+            fix_add(a, grp, flags, huge, nan);
+            acc[tid] = a;
+        }
+        grp = 0.0;
+    };
     const bool pin = what != 3 && gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
     const bool col_interior = own && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
     const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
@@ -276,21 +296,21 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // (loads and stores share one counter, which the compiler can only wait
     // to zero when both kinds are pending).
     // (the trial values of the pending plane are window slot 0 by then)
-    struct Pending { bool on; unsigned so; float n[3]; };
+    struct Pending { bool on; float n[3]; };
     Pending pend;
     pend.on = false;
-    pend.so = 0u;
-    auto store_pending = [&]() {
+    // so: byte offset of the pending plane (wave-uniform)
+    auto store_pending = [&](unsigned so) {
         if (pend.on) {
             if (has_next) {
-                bstore(r_next, pend.n[0], pv, pend.so);
-                bstore(r_next, pend.n[1], pv, pend.so + cn4);
-                bstore(r_next, pend.n[2], pv, pend.so + 2u * cn4);
+                bstore(r_next, pend.n[0], pv, so);
+                bstore(r_next, pend.n[1], pv, so + cn4);
+                bstore(r_next, pend.n[2], pv, so + 2u * cn4);
             }
             if (has_trial) {
-                bstore(r_trial, col[0][0], pv, pend.so);
-                bstore(r_trial, col[0][1], pv, pend.so + cn4);
-                bstore(r_trial, col[0][2], pv, pend.so + 2u * cn4);
+                bstore(r_trial, col[0][0], pv, so);
+                bstore(r_trial, col[0][1], pv, so + cn4);
+                bstore(r_trial, col[0][2], pv, so + 2u * cn4);
             }
         }
         pend.on = false;
@@ -319,7 +339,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             }
             commit(sa == 0 ? 2 : sa - 1, vz);
         }
-        store_pending();
+        store_pending(4u * (unsigned)max(zz - 2, 0) * plane);   // plane zz-2, stage B of the last step
         fetch(zz + 3, rn);
         fetch_tm(zz + 1, nxt);
 
@@ -337,7 +357,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             }
             const bool count = own && zz >= z0 && zz < z1;
             CT W[9];
-            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Esum, mx, W);
+            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
             const int pz = zz & 1;
             if (what != 2) {
 #pragma unroll
@@ -350,11 +370,11 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 #pragma unroll
             for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
         }
+        if (zz >= z0 && zz < z1 && ((zz & 3) == 3 || zz == nz - 1)) flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
 
         // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
-        pend.so = 4u * (unsigned)max(zb, 0) * plane;   // wave-uniform (a scalar offset of the stores)
         if (own && pin && zb >= z0 && zb < z1) {
             const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
@@ -368,7 +388,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                 f0 = (float)(cix * dxx - ciy * dxy + ciz * (wz_m[0] - wz_p[0]));
                 f1 = (float)(-cix * dyx + ciy * dyy - ciz * (wz_m[1] - wz_p[1]));
                 f2 = (float)(cix * dzx - ciy * dzy + ciz * (wz_m[2] - wz_p[2]));
-                Gsum += (double)f0 * f0 + (double)f1 * f1 + (double)f2 * f2;
+                Ggrp += (double)f0 * f0 + (double)f1 * f1 + (double)f2 * f2;
             }
             // trial value of this voxel on plane zb: window slot 1 (planes zz-2 .. zz+2);
             // stored at the start of the next step
@@ -376,6 +396,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             pend.on = true;
             pend.n[0] = ux - eta_next * f0; pend.n[1] = uy - eta_next * f1; pend.n[2] = uz - eta_next * f2;
         }
+        if (zb >= z0 && zb < z1 && ((zb & 3) == 3 || zb == nz - 1)) flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { wz_m[a] = wz_c[a]; wz_c[a] = wz_p[a]; }
         __syncthreads();
@@ -395,12 +416,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // the last step's outputs: shift the window as a next step would
 #pragma unroll
     for (int c = 0; c < 3; ++c) col[0][c] = col[1][c];
-    store_pending();
+    store_pending(4u * (unsigned)max(z1 - 1, 0) * plane);
+    // a chunk that does not end on a group boundary (not planned so) keeps
+    // its remainder
+    flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
+    flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
 }
 
 template <typename CT, int TX, int TY, int MINW>
 __global__ void __launch_bounds__((StreamGeom<TX, TY>::NT), MINW)
-k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
+k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, int parity)
 {
     constexpr int NT = StreamGeom<TX, TY>::NT;
     __shared__ StreamSmem<TX, TY> S;
@@ -408,29 +433,23 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    Ctrl *C = I.ctrl;
+    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, tl.tile == 0, &S.ctrl, S.acc_w);
+    __syncthreads();
+    const Ctrl *C = &S.ctrl;
     if (!uni(C->active)) return;
     TileCoord tc = decode_tile<TX, TY>(I, tl.tile);
     tc.x0 = uni(tc.x0); tc.y0 = uni(tc.y0); tc.z0 = uni(tc.z0); tc.z1 = uni(tc.z1);
 
-    double Esum = 0.0, Gsum = 0.0;
+    unsigned flags = 0;
     float mx = 0.0f;
     if (uni(C->mode) == TRIAL_INTERP)
-        stream_pass_body<CT, TX, TY, true>(S, I, C, tc, Esum, mx, Gsum);
+        stream_pass_body<CT, TX, TY, true>(S, I, C, tc, flags, mx);
     else
-        stream_pass_body<CT, TX, TY, false>(S, I, C, tc, Esum, mx, Gsum);
+        stream_pass_body<CT, TX, TY, false>(S, I, C, tc, flags, mx);
 
-    block_reduce3<NT>(Esum, mx, Gsum, S.red);
-    if (arrive<NT>(I, tl.tile, Esum, mx, Gsum, &S.flag)) {
-        double E, G;
-        float m;
-        fold_partials<NT>(I, E, m, G, S.red);
-        if (threadIdx.x == 0) {
-            grad_controller(I, C, E, m, G);
-            publish_status(I, C);
-            __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    const bool own = threadIdx.x < StreamGeom<TX, TY>::NOWN;
+    add_pass_sums<NT>(I, uni(C->acc_cur), own ? S.accE[threadIdx.x] : (u128)0, own ? S.accG[threadIdx.x] : (u128)0,
+                      mx, flags, S.red);
 }
 
 // One (precision, tile) variant per compilation of this file: the register
@@ -443,12 +462,12 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
 #define DT_PASTE2(a, b) a##b
 #define DT_PASTE(a, b) DT_PASTE2(a, b)
 
-hipError_t DT_PASTE(stream_launch_, DT_STREAM_NAME)(const Inst *insts, const Tile *tiles, int nblocks,
+hipError_t DT_PASTE(stream_launch_, DT_STREAM_NAME)(const Inst *insts, const Tile *tiles, int nblocks, int parity,
                                                      hipStream_t st)
 {
     constexpr int NT = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NT;
     hipLaunchKernelGGL((k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>), dim3(nblocks),
-                       dim3(NT), 0, st, insts, tiles);
+                       dim3(NT), 0, st, insts, tiles, parity);
     return hipGetLastError();
 }
 

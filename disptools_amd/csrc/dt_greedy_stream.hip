@@ -6,8 +6,9 @@
 // where the outer face keeps the output buffer's contents (the reference never
 // writes it), evaluates the Jacobian of the trial (region map of
 // src/jacobian.c:70-242), its voxel error (src/headers/error.h:128-142) into
-// the new error map, the FP64 energy sum and masked max, and the last block
-// runs the greedy controller (displacement_field_greedy.c:85-247).  With the
+// the new error map, the exact energy sum and masked max; the greedy
+// controller (displacement_field_greedy.c:85-247) advances at the start of
+// the next launch, in every block, on these sums (pass_begin).  With the
 // step off (the evaluation of the initial guess) the trial is u_base itself.
 //
 // A block owns a TX x TY column tile and streams z.  Per plane step zz:
@@ -51,8 +52,9 @@ struct GreedySmem {
     using G = GreedyGeom<TX, TY>;
     float es[2][G::EP];            // error-map planes by parity
     float us[3][3][G::RP];         // trial planes, ring of three
-    double red[3 * (G::NT / 64)];
-    int flag;
+    SumsWave red[G::NT / 64];
+    Ctrl ctrl;                     // this pass's control state (pass_begin)
+    unsigned long long acc_w[BEGIN_ACC_WORDS];   // the instance's accumulators (pass_begin)
 };
 
 // Region coordinates (rx, ry) of the r-th point of the region ring (outside
@@ -75,10 +77,14 @@ __device__ __forceinline__ void ring3_point(int r, int &ex, int &ey)
     else { const int q = r - 2 * EX; ey = 1 + q / 2; ex = (q & 1) ? TX + 5 : 0; }
 }
 
+// The energy is summed exactly (fixed point, dt_device.hpp): its terms are
+// the double sums of e^2 over groups of four planes of one column, starting
+// at z = 0 mod 4, so the sum does not depend on the tiling.
 template <typename CT, int TX, int TY>
 __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I, const Ctrl *C, const TileCoord tc,
-                                            double &Esum, float &mx)
+                                            u128 &Esum, unsigned &flags, float &mx)
 {
+    double Egrp = 0.0;
     using G = GreedyGeom<TX, TY>;
     constexpr int RX = G::RX, RP = G::RP, EX = G::EX, NRING2 = G::NRING2, NRING3 = G::NRING3;
 
@@ -303,9 +309,13 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
                                                        : jacobian_delta<CT>(P, gxs[0], gys[0], zz, sm, col[2]);
             const bool in_mask = cur.m != 0;
             const float e = voxel_error_delta(dJ, cur.t - 1.0f, in_mask, P.tol);
-            Esum += (double)e * (double)e;
+            Egrp += (double)e * (double)e;
             if (in_mask) mx = fmaxf(mx, fabsf(e));
             bstore(r_enew, e, 4u * vs[0], 4u * (unsigned)zz * plane);
+        }
+        if (zz >= z0 && zz < z1 && ((zz & 3) == 3 || zz == nz - 1)) {
+            fix_add(Esum, Egrp, flags, ACC_E_HUGE, ACC_E_NAN);
+            Egrp = 0.0;
         }
         // slide the error window: planes zz+2, zz+3 (zz+4 arrives next step)
 #pragma unroll
@@ -315,45 +325,41 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
 
 #pragma unroll 1
     for (int zz = z0 - 4; zz < z1; ++zz) step(zz);
+    fix_add(Esum, Egrp, flags, ACC_E_HUGE, ACC_E_NAN);   // a chunk not ending on a group boundary
 }
 
 template <typename CT, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY, 2)
-k_greedy_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
+k_greedy_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, int parity)
 {
     constexpr int NT = TX * TY;
     __shared__ GreedySmem<TX, TY> S;
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    Ctrl *C = I.ctrl;
+    if (threadIdx.x < 64) pass_begin<DT_GREEDY>(I, parity, tl.tile == 0, &S.ctrl, S.acc_w);
+    __syncthreads();
+    const Ctrl *C = &S.ctrl;
     if (!uni(C->active)) return;
     TileCoord tc = decode_tile<TX, TY>(I, tl.tile);
     tc.x0 = uni(tc.x0); tc.y0 = uni(tc.y0); tc.z0 = uni(tc.z0); tc.z1 = uni(tc.z1);
-    double Esum = 0.0, dummy = 0.0;
+    u128 Esum = 0;
+    unsigned flags = 0;
     float mx = 0.0f;
-    greedy_body<CT, TX, TY>(S, I, C, tc, Esum, mx);
-    block_reduce3<NT>(Esum, mx, dummy, S.red);
-    if (arrive<NT>(I, tl.tile, Esum, mx, 0.0, &S.flag)) {
-        double E, Gs;
-        float m;
-        fold_partials<NT>(I, E, m, Gs, S.red);
-        if (threadIdx.x == 0) {
-            greedy_controller(I, C, E, m, 2);
-            publish_status(I, C);
-            __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    greedy_body<CT, TX, TY>(S, I, C, tc, Esum, flags, mx);
+    add_pass_sums<NT>(I, uni(C->acc_cur), Esum, (u128)0, mx, flags, S.red);
 }
 
 template <int TX, int TY>
-static hipError_t launch_greedy_fused(int precision, const Inst *insts, const Tile *tiles, int nblocks,
+static hipError_t launch_greedy_fused(int precision, const Inst *insts, const Tile *tiles, int nblocks, int parity,
                                       hipStream_t st)
 {
     if (precision == 64)
-        hipLaunchKernelGGL((k_greedy_stream<double, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles);
+        hipLaunchKernelGGL((k_greedy_stream<double, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles,
+                           parity);
     else
-        hipLaunchKernelGGL((k_greedy_stream<float, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles);
+        hipLaunchKernelGGL((k_greedy_stream<float, TX, TY>), dim3(nblocks), dim3(TX * TY), 0, st, insts, tiles,
+                           parity);
     return hipGetLastError();
 }
 
@@ -368,20 +374,17 @@ static int greedy_fused_blocks_per_cu(int precision)
 }
 
 hipError_t launch_greedy_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
-                              hipStream_t st)
+                              int parity, hipStream_t st)
 {
-    if (variant == GREEDY_FUSED) return launch_greedy_fused<32, 16>(precision, insts, tiles, nblocks, st);
-    if (variant == GREEDY_FUSED_WIDE) return launch_greedy_fused<32, 32>(precision, insts, tiles, nblocks, st);
-    hipError_t r = launch_greedy_step(insts, tiles, nblocks, st);
-    if (r != hipSuccess) return r;
-    return launch_greedy_eval(precision, insts, tiles, nblocks, st);
+    if (variant == GREEDY_FUSED_WIDE)
+        return launch_greedy_fused<32, 32>(precision, insts, tiles, nblocks, parity, st);
+    return launch_greedy_fused<32, 16>(precision, insts, tiles, nblocks, parity, st);
 }
 
 int greedy_pass_blocks_per_cu(int precision, int variant)
 {
-    if (variant == GREEDY_FUSED) return greedy_fused_blocks_per_cu<32, 16>(precision);
     if (variant == GREEDY_FUSED_WIDE) return greedy_fused_blocks_per_cu<32, 32>(precision);
-    return 4;
+    return greedy_fused_blocks_per_cu<32, 16>(precision);
 }
 
 }  // namespace dt

@@ -88,7 +88,7 @@ struct Plan {
     int precision = 32;      // stencil arithmetic: 32 or 64 bits
     int variant = 0;         // pass variant of the algorithm (column tile shape)
     int nx = 0, ny = 0, nz = 0;
-    int tx = TILE_X, ty = TILE_Y, tz = 1;
+    int tx = 64, ty = 12, tz = 4;
     int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
     long long n = 0;
 };
@@ -119,24 +119,30 @@ int precision_from_env();
 // from device memory).  For testing that path.
 bool status_words_from_env();
 // Tiling of an instance; `variant` is the pass variant of the algorithm
-// (gradient: GRAD_VARIANTS, greedy: 0 or GREEDY_FUSED); tz <= 0 picks the
-// depth for a launch of this instance alone (choose_tz).
+// (gradient: 0 .. GRAD_VARIANTS-1, greedy: GREEDY_FUSED or GREEDY_FUSED_WIDE);
+// tz <= 0 picks the depth for a launch of this instance alone (choose_tz).
+// Depths are multiples of 4 (the groups of planes of the exact sums).
 Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant, int tz);
 
 // Device bytes of an instance's solver state besides target, mask and the
-// caller's field (extra field buffers, error maps, tile partials, trace; the
-// control block and counter too when `own_ctrl`).
+// caller's field (extra field buffers, error maps, trace; the two control
+// copies and the accumulators too when `own_ctrl`).
 size_t scratch_bytes(const Plan &P, long long trace_cap, bool own_ctrl);
 
 // Fills the instance record: geometry, parameters, the caller's buffers and
 // the solver state carved from `cv` (same order and sizes as scratch_bytes).
-// ctrl / counter are carved too when null.
+// ctrl (two copies) and acc (ACC_SLOTS * ACC_COPIES) are carved too when null.
 Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, const float *target,
                const uint8_t *mask, float *field, Carver &cv, double *trace, long long trace_cap, Ctrl *ctrl,
-               unsigned int *counter);
+               Acc *acc);
 
-// Control block of a solve that starts from the initial guess in buf[0].
+// Control block of a solve that starts from the initial guess in buf[0]
+// (both copies start from it; the accumulators start at zero).
 Ctrl initial_ctrl(int algorithm, const dt_params &p);
+
+// The newer of an instance's two control copies (the one that records the end
+// of the solve, or the one with more evaluated passes).
+const Ctrl &latest_ctrl(const Ctrl (&c)[2]);
 
 // Appends the tiles of `segments` (instance index, tile count) to `out`, in
 // the order that gives block b (dispatched to XCD b % 8) a contiguous run of

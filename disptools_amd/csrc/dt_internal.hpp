@@ -31,9 +31,13 @@ enum Phase : int {
     PH_DONE = 3,
 };
 
-// Per-solve control block, resident in device memory.  The pass kernels read
-// the first group of fields; the controller (one thread of the last block of
-// each pass) updates everything.
+// Per-solve control block, resident in device memory in two copies (one per
+// pass parity).  At the start of a pass, one thread of every block reads the
+// copy of the pass's parity and the sums of the previous pass and advances
+// the controller (every block reaches the same state: the inputs are the
+// same); the instance's first tile writes the new state into the other copy,
+// which the next pass reads.  The pass kernels then use the first group of
+// fields.
 struct Ctrl {
     // ---- read by the pass kernels ----
     int active;      // 0 once the solve has terminated (passes exit at once)
@@ -49,6 +53,10 @@ struct Ctrl {
     int e_cur;       // greedy: error buffer the step reads
     int e_new;       // greedy: error buffer the evaluation writes
     float eta;       // step length of the trial being evaluated
+    int acc_cur;     // accumulator slot this pass adds its sums into
+    int acc_prev;    // slot holding the previous pass's sums
+    int acc_zero;    // slot the instance's first tile clears during this pass
+    int evaluated;   // a trial has been evaluated since the state was last advanced
     // ---- controller state ----
     int phase;
     int base;        // accepted iterate
@@ -69,6 +77,26 @@ struct Ctrl {
 
 static_assert(sizeof(Ctrl) % 8 == 0, "the control block is published in 8-byte words");
 
+// Exact sums of one pass.  A sum of squares is kept as an unsigned 128-bit
+// fixed-point number with quantum 2^-64, split into four 32-bit limbs, each
+// limb added into a 64-bit word: integer addition in any order gives the same
+// total, so the sums do not depend on how a volume is tiled.  Blocks add
+// into the copy of their XCD (block index mod ACC_COPIES) to spread the
+// atomics; a slot is ACC_COPIES copies, an instance has ACC_SLOTS slots
+// (the pass adding, the previous pass's sums, the one being cleared).
+constexpr int ACC_COPIES = 8;
+constexpr int ACC_SLOTS = 3;
+struct Acc {
+    unsigned long long e[4];   // sum of e^2 over all voxels, limbs
+    unsigned long long g[4];   // sum of |g|^2, limbs
+    unsigned int mx;           // max |e| over the mask (bits of a non-negative float)
+    unsigned int flags;        // ACC_E_* / ACC_G_* below
+};
+constexpr unsigned ACC_E_HUGE = 1u;   // an e^2 term at or above 2^32, or infinite
+constexpr unsigned ACC_E_NAN = 2u;
+constexpr unsigned ACC_G_HUGE = 4u;
+constexpr unsigned ACC_G_NAN = 8u;
+
 // One solve instance (a volume with its parameters and buffers).
 struct Inst {
     int nx, ny, nz;
@@ -82,10 +110,8 @@ struct Inst {
     const uint8_t *mask;
     float *buf[NBUF];
     float *err[2];           // greedy error maps
-    float *wmap;             // penalised weight per voxel (two-kernel gradient pass)
-    double *partials;        // [tiles][4]: sum e^2, max |e| (masked), sum |g|^2, spare
-    unsigned int *counter;   // blocks of this instance that finished the pass
-    Ctrl *ctrl;
+    Acc *acc;                // [ACC_SLOTS][ACC_COPIES] exact sums of the passes
+    Ctrl *ctrl;              // [2]: control block by pass parity
     double *trace;           // [trace_cap][4]
     long long trace_cap;
     unsigned long long *status;   // host-coherent completion word (null: none)

@@ -1,58 +1,15 @@
-// dt_kernels.hip -- gfx950 kernels of the displacement-generation path.
+// dt_kernels.hip -- gfx950 kernels of the displacement-generation path
+// besides the solver passes (dt_grad_stream.hip, dt_greedy_stream.hip):
 //
 //   k_jacobian     det(I + grad u) over a volume (reference jacobian, src/jacobian.c:70-242)
-//   k_grad_pass    one fused pass of the gradient solver: trial field -> Jacobian ->
-//                  voxel error -> adjoint gradient -> speculative next candidate,
-//                  with FP64 energy sums and the line-search controller in the
-//                  last block (src/displacement_field_gradient.c:26-431)
-//   k_greedy_step  greedy update of the interior (src/displacement_field_greedy.c:23-67)
-//   k_greedy_eval  Jacobian + error of a greedy trial + controller (:85-247)
 //   k_regularise   J = max(J, epsilon) (src/jacobian.c:29-55)
-//
-// All volume kernels work on tiles of TX x TY columns and TZ planes; a launch
-// walks a tile table that may hold tiles of several instances (batched solves).
+//   k_take_result  the solve's result into the caller's field buffer
 #include <math.h>
 
 #include "dt_device.hpp"
 #include "dt_kernels.hpp"
 
 namespace dt {
-
-// ---------------------------------------------------------------------------
-// Greedy kernels
-// ---------------------------------------------------------------------------
-
-// u_new = u_old + (eta d_c / 2)(e(p + e_c) - e(p - e_c)) on the interior
-// (displacement_field_greedy.c:36-63); the outer face of u_new is not written.
-template <int TX, int TY>
-__global__ void __launch_bounds__(TX * TY)
-k_greedy_step(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
-{
-    const Tile tl = tiles[blockIdx.x];
-    if (tl.inst < 0) return;
-    const Inst &I = insts[tl.inst];
-    const Ctrl *C = I.ctrl;
-    if (!C->active || !C->step_on) return;
-    const TileCoord tc = decode_tile<TX, TY>(I, tl.tile);
-    const int gx = tc.x0 + (int)threadIdx.x % TX, gy = tc.y0 + (int)threadIdx.x / TX;
-    const int nx = I.nx, ny = I.ny, nz = I.nz;
-    if (gx < 1 || gx > nx - 2 || gy < 1 || gy > ny - 2) return;
-    const float *__restrict__ uo = I.buf[C->base];
-    float *__restrict__ un = I.buf[C->g_new];
-    const float *__restrict__ e = I.err[C->e_cur];
-    const float eta = C->eta;
-    const float ddx = eta * I.dx * .5f, ddy = eta * I.dy * .5f, ddz = eta * I.dz * .5f;
-    const long long n = I.n, sy = nx, sz = (long long)nx * ny;
-    for (int z = max(tc.z0, 1); z < min(tc.z1, nz - 1); ++z) {
-        const long long i = ((long long)z * ny + gy) * nx + gx;
-        const float sx_ = ddx * (e[i + 1] - e[i - 1]);
-        const float sy_ = ddy * (e[i + sy] - e[i - sy]);
-        const float sz_ = ddz * (e[i + sz] - e[i - sz]);
-        un[i] = uo[i] + sx_;
-        un[n + i] = uo[n + i] + sy_;
-        un[2 * n + i] = uo[2 * n + i] + sz_;
-    }
-}
 
 // Jacobian at one voxel read straight from global memory (region map of src/jacobian.c).
 template <typename CT>
@@ -81,51 +38,6 @@ __device__ __forceinline__ float jacobian_global(const float *__restrict__ f, lo
         }
     }
     return (float)det_plus_identity<CT>(d);
-}
-
-// Jacobian + error of the greedy trial buf[g_new] (or of the initial guess),
-// energy sums, controller.
-template <typename CT, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY)
-k_greedy_eval(const Inst *__restrict__ insts, const Tile *__restrict__ tiles)
-{
-    constexpr int NT = TX * TY;
-    __shared__ double red[3 * (NT / 64)];
-    __shared__ int flag;
-    const Tile tl = tiles[blockIdx.x];
-    if (tl.inst < 0) return;
-    const Inst &I = insts[tl.inst];
-    Ctrl *C = I.ctrl;
-    if (!C->active) return;
-    const TileCoord tc = decode_tile<TX, TY>(I, tl.tile);
-    const int gx = tc.x0 + (int)threadIdx.x % TX, gy = tc.y0 + (int)threadIdx.x / TX;
-    const int nx = I.nx, ny = I.ny, nz = I.nz;
-    const float *__restrict__ f = I.buf[C->phase == PH_INIT ? C->base : C->g_new];
-    float *__restrict__ e = I.err[C->e_new];
-    double Esum = 0.0, dummy = 0.0;
-    float mx = 0.0f;
-    if (gx < nx && gy < ny) {
-        for (int z = tc.z0; z < tc.z1; ++z) {
-            const float J = jacobian_global<CT>(f, I.n, gx, gy, z, nx, ny, nz, I.ix, I.iy, I.iz);
-            const long long i = ((long long)z * ny + gy) * nx + gx;
-            const bool in_mask = I.mask[i] != 0;
-            const float ev = voxel_error(J, I.target[i], in_mask, I.p.tolerance);
-            e[i] = ev;
-            Esum += (double)ev * (double)ev;
-            if (in_mask) mx = fmaxf(mx, fabsf(ev));
-        }
-    }
-    block_reduce3<NT>(Esum, mx, dummy, red);
-    if (arrive<NT>(I, tl.tile, Esum, mx, 0.0, &flag)) {
-        double E, G;
-        float m;
-        fold_partials<NT>(I, E, m, G, red);
-        if (threadIdx.x == 0) {
-            greedy_controller(I, C, E, m, 3);
-            publish_status(I, C);
-            __hip_atomic_store(I.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -159,7 +71,10 @@ __global__ void __launch_bounds__(256)
 k_take_result(const Inst *__restrict__ inst)
 {
     const Inst &I = *inst;
-    const int rb = __builtin_amdgcn_readfirstlane(I.ctrl->result_buf);
+    // the control copy that records the end of the solve (both, once a pass
+    // after the end has run; they then agree)
+    const Ctrl *c = I.ctrl[1].phase == PH_DONE && I.ctrl[0].phase != PH_DONE ? I.ctrl + 1 : I.ctrl;
+    const int rb = __builtin_amdgcn_readfirstlane(c->result_buf);
     if (rb <= 0 || rb >= NBUF) return;
     const float *__restrict__ src = I.buf[rb];
     float *__restrict__ dst = I.buf[0];
@@ -205,22 +120,43 @@ hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_greedy_step(const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st)
+// Gradient pass variants, one object per (precision, tile) (dt_grad_stream.hip);
+// variant k is entry k of this list.
+#define DT_STREAM_VARIANTS(X) X(0, 32x16_w3) X(1, 32x20_w3) X(2, 64x8_w3) X(3, 32x24_w3) X(4, 64x12_w3)
+#define DT_DECLARE_STREAM(k, name)                                                                       \
+    hipError_t stream_launch_f32_##name(const Inst *insts, const Tile *tiles, int nblocks, int parity,  \
+                                        hipStream_t st);                                                \
+    hipError_t stream_launch_f64_##name(const Inst *insts, const Tile *tiles, int nblocks, int parity,  \
+                                        hipStream_t st);                                                \
+    int stream_occupancy_f32_##name();                                                                   \
+    int stream_occupancy_f64_##name();
+DT_STREAM_VARIANTS(DT_DECLARE_STREAM)
+
+hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const Tile *tiles, int nblocks,
+                            int parity, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_greedy_step<TILE_X, TILE_Y>), dim3(nblocks), dim3(TILE_X * TILE_Y), 0, st,
-                       insts, tiles);
-    return hipGetLastError();
+    const bool d = precision == 64;
+#define DT_LAUNCH_CASE(k, name)                                                            \
+    case k:                                                                                \
+        return d ? stream_launch_f64_##name(insts, tiles, nblocks, parity, st)             \
+                 : stream_launch_f32_##name(insts, tiles, nblocks, parity, st);
+    switch (variant) {
+        DT_STREAM_VARIANTS(DT_LAUNCH_CASE)
+    default: return hipErrorInvalidValue;
+    }
+#undef DT_LAUNCH_CASE
 }
 
-hipError_t launch_greedy_eval(int precision, const Inst *insts, const Tile *tiles, int nblocks, hipStream_t st)
+int grad_pass_blocks_per_cu(int precision, int variant)
 {
-    if (precision == 64)
-        hipLaunchKernelGGL((k_greedy_eval<double, TILE_X, TILE_Y>), dim3(nblocks), dim3(TILE_X * TILE_Y), 0,
-                           st, insts, tiles);
-    else
-        hipLaunchKernelGGL((k_greedy_eval<float, TILE_X, TILE_Y>), dim3(nblocks), dim3(TILE_X * TILE_Y), 0,
-                           st, insts, tiles);
-    return hipGetLastError();
+    const bool d = precision == 64;
+#define DT_OCC_CASE(k, name) \
+    case k: return d ? stream_occupancy_f64_##name() : stream_occupancy_f32_##name();
+    switch (variant) {
+        DT_STREAM_VARIANTS(DT_OCC_CASE)
+    default: return 1;
+    }
+#undef DT_OCC_CASE
 }
 
 }  // namespace dt

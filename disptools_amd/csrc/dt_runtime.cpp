@@ -80,13 +80,13 @@ struct Arena {
     hipEvent_t kev[2] = {nullptr, nullptr};     // timing events of a solve's passes
 };
 
-// Instance record, control block and arrival counter of a single-volume
-// solve; its tile table follows in the same allocation, so one copy stages
-// all of them.
+// Instance record, control copies and accumulators of a single-volume solve;
+// its tile table follows in the same allocation, so one copy stages all of
+// them.
 struct alignas(256) SolveHead {
     Inst inst;
-    Ctrl ctrl;
-    unsigned int counter;
+    Ctrl ctrl[2];
+    Acc acc[ACC_SLOTS * ACC_COPIES];
 };
 
 static int arena_stage(Arena &a, size_t bytes)
@@ -208,16 +208,21 @@ size_t scratch_bytes(const Plan &P, long long trace_cap, bool own_ctrl)
     const int extra_fields = P.algorithm == DT_GRADIENT ? NBUF - 1 : 1;
     for (int i = 0; i < extra_fields; ++i) add(3 * (size_t)n * sizeof(float));
     if (P.algorithm == DT_GREEDY) { add((size_t)n * sizeof(float)); add((size_t)n * sizeof(float)); }
-    if (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE && P.variant < STREAM_BASE) add((size_t)n * sizeof(float));
-    add((size_t)P.ntiles * 4 * sizeof(double));
     if (trace_cap > 0) add((size_t)trace_cap * 4 * sizeof(double));
-    if (own_ctrl) { add(sizeof(unsigned int)); add(sizeof(Ctrl)); }
+    if (own_ctrl) { add(ACC_SLOTS * ACC_COPIES * sizeof(Acc)); add(2 * sizeof(Ctrl)); }
     return align_up(b, 256);
+}
+
+const Ctrl &latest_ctrl(const Ctrl (&c)[2])
+{
+    if (c[1].phase == PH_DONE && c[0].phase != PH_DONE) return c[1];
+    if (c[0].phase == PH_DONE) return c[0];
+    return c[1].evaluations > c[0].evaluations ? c[1] : c[0];
 }
 
 Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, const float *target,
                const uint8_t *mask, float *field, Carver &cv, double *trace, long long trace_cap, Ctrl *ctrl,
-               unsigned int *counter)
+               Acc *acc)
 {
     Inst h{};
     h.nx = P.nx; h.ny = P.ny; h.nz = P.nz;
@@ -240,8 +245,6 @@ Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, 
         h.err[0] = cv.take<float>((size_t)n);
         h.err[1] = cv.take<float>((size_t)n);
     }
-    h.wmap = (P.algorithm == DT_GRADIENT && P.variant >= SPLIT_BASE && P.variant < STREAM_BASE) ? cv.take<float>((size_t)n) : nullptr;
-    h.partials = cv.take<double>((size_t)P.ntiles * 4);
     if (trace_cap > 0) {
         double *t = cv.take<double>((size_t)trace_cap * 4);
         h.trace = trace ? t : nullptr;
@@ -250,8 +253,8 @@ Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, 
         h.trace = nullptr;
         h.trace_cap = 0;
     }
-    h.counter = counter ? counter : cv.take<unsigned int>(1);
-    h.ctrl = ctrl ? ctrl : cv.take<Ctrl>(1);
+    h.acc = acc ? acc : cv.take<Acc>(ACC_SLOTS * ACC_COPIES);
+    h.ctrl = ctrl ? ctrl : cv.take<Ctrl>(2);
     return h;
 }
 
@@ -277,6 +280,11 @@ Ctrl initial_ctrl(int algorithm, const dt_params &p)
         c.e_new = 0;
     }
     c.result_buf = 0;
+    // the first pass adds into slot 0 and clears slot 1 (already clear)
+    c.acc_cur = 0;
+    c.acc_zero = 1;
+    c.acc_prev = 2;
+    c.evaluated = 0;
     return c;
 }
 
@@ -327,7 +335,8 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int
     const long long cols = (long long)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
     int best = std::max(1, std::min(nz, 16));
     double best_cost = 1e300;
-    for (int tz = 4; tz <= std::max(4, nz); ++tz) {
+    // multiples of 4: z-chunks start on the groups of planes of the exact sums
+    for (int tz = 4; tz <= std::max(4, nz + 3); tz += 4) {
         const long long blocks = cols * ((nz + tz - 1) / tz);
         const long long per = shared_cu ? (blocks + cus - 1) / cus : (blocks + resident - 1) / resident;
         const double cost = (double)per * (std::min(tz, nz) + overhead);
@@ -336,13 +345,10 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int
     return best;
 }
 
-// Default pass variant: the single-barrier streaming pass on 64 x 12 column
-// tiles with the tile ring in waves of its own (measured fastest at 128^3
-// among the compiled variants).  Engine lanes use it; single solves start
-// from it and switch tile by model_grad_variant; DISPTOOLS_GRAD_VARIANT
-// overrides both.
-constexpr int DEFAULT_GRAD_VARIANT = STREAM_BASE + 4;
-
+// Default pass variant (DEFAULT_GRAD_VARIANT): 64 x 12 column tiles with the
+// tile ring in waves of its own (measured fastest at 128^3 among the compiled
+// variants).  Engine lanes use it; single solves start from it and switch
+// tile by model_grad_variant; DISPTOOLS_GRAD_VARIANT overrides both.
 int variant_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_GRAD_VARIANT");
@@ -351,13 +357,11 @@ int variant_from_env()
     return (v >= 0 && v < GRAD_VARIANTS) ? v : DEFAULT_GRAD_VARIANT;
 }
 
-// Greedy pass variant from DISPTOOLS_GREEDY_VARIANT: 0 = step + evaluation
-// kernels, 1 = fused pass on 32 x 16 tiles, 2 = fused pass on 32 x 32 tiles;
-// -1 when unset.
+// Greedy pass variant from DISPTOOLS_GREEDY_VARIANT: 1 = 32 x 16 column
+// tiles, 2 = 32 x 32; -1 when unset.
 int greedy_variant_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_GREEDY_VARIANT");
-    if (s && std::strcmp(s, "0") == 0) return 0;
     if (s && std::strcmp(s, "1") == 0) return GREEDY_FUSED;
     if (s && std::strcmp(s, "2") == 0) return GREEDY_FUSED_WIDE;
     return -1;
@@ -374,10 +378,8 @@ static int greedy_solve_variant(int nx, int ny, int precision)
     return (precision == 32 && (long long)nx * ny >= 8192) ? GREEDY_FUSED_WIDE : GREEDY_FUSED;
 }
 
-// Variants the autotuner times: the fused passes (the two-kernel variants
-// need a weight map and are slower everywhere measured).
-static const int TUNE_CANDIDATES[] = {1, STREAM_BASE, STREAM_BASE + 1, STREAM_BASE + 2, STREAM_BASE + 3,
-                                      STREAM_BASE + 4};
+// Variants the autotuner times: every column tile of the gradient pass.
+static const int TUNE_CANDIDATES[] = {0, 1, 2, 3, 4};
 
 struct TuneKey {
     int dev, nx, ny, nz;
@@ -404,8 +406,8 @@ bool status_words_from_env()
 static int plan_tz(const Plan &P)
 {
     const int cus = device_cus();
-    const bool fused_greedy = P.algorithm == DT_GREEDY && greedy_fused(P.variant);
-    const bool stream = fused_greedy || (P.algorithm == DT_GRADIENT && P.variant >= STREAM_BASE);
+    const bool fused_greedy = P.algorithm == DT_GREEDY;
+    const bool stream = true;
     const int per_cu = cached_query(1 + P.algorithm, P.precision, P.variant, [&] {
         return P.algorithm == DT_GRADIENT ? grad_pass_blocks_per_cu(P.precision, P.variant)
                                           : greedy_pass_blocks_per_cu(P.precision, P.variant);
@@ -418,13 +420,13 @@ Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant
     Plan P;
     P.algorithm = algorithm;
     P.precision = precision;
-    P.variant = algorithm == DT_GRADIENT ? variant : (greedy_fused(variant) ? variant : 0);
+    P.variant = algorithm == DT_GRADIENT ? variant : (variant == GREEDY_FUSED_WIDE ? GREEDY_FUSED_WIDE : GREEDY_FUSED);
     P.nx = nx; P.ny = ny; P.nz = nz;
     P.n = (long long)nx * ny * nz;
     if (algorithm == DT_GRADIENT) grad_variant_dims(P.variant, P.tx, P.ty);
     else greedy_variant_dims(P.variant, P.tx, P.ty);
-    P.tz = tz > 0 ? std::min(tz, nz)
-                  : plan_tz(P);
+    // a given depth is rounded up to a multiple of 4
+    P.tz = tz > 0 ? std::min((tz + 3) / 4 * 4, nz) : plan_tz(P);
     P.ntx = (nx + P.tx - 1) / P.tx;
     P.nty = (ny + P.ty - 1) / P.ty;
     P.ntz = (nz + P.tz - 1) / P.tz;
@@ -531,28 +533,30 @@ static int tune_gradient(int nx, int ny, int nz, float dx, float dy, float dz, i
         Tile *d_tiles = cv.take<Tile>((size_t)P.ntiles);
         std::vector<Tile> tiles;
         append_tiles_xcd(tiles, {{0, P.ntiles}});
-        const Ctrl c = initial_ctrl(DT_GRADIENT, p);
+        const Ctrl c0[2] = {initial_ctrl(DT_GRADIENT, p), initial_ctrl(DT_GRADIENT, p)};
         float ms = 0.0f;
         if (hipMemsetAsync(d_field, 0, 12 * (size_t)n, st) != hipSuccess ||
             hipMemcpyAsync(d_inst, &h, sizeof(Inst), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, st) !=
                 hipSuccess ||
-            hipMemcpyAsync(h.ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemsetAsync(h.counter, 0, sizeof(unsigned int), st) != hipSuccess) {
+            hipMemcpyAsync(h.ctrl, c0, sizeof c0, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemsetAsync(h.acc, 0, ACC_SLOTS * ACC_COPIES * sizeof(Acc), st) != hipSuccess) {
             set_error(__LINE__, __FILE__, __func__, "cannot stage a tuning instance");
             rc = -1;
             break;
         }
-        for (int i = 0; i < warm && !rc; ++i)
-            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, st) != hipSuccess) rc = -1;
+        int launched = 0;   // the pass index's parity selects the control copy
+        for (int i = 0; i < warm && !rc; ++i, ++launched)
+            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, launched & 1, st) != hipSuccess) rc = -1;
         if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = -1;
-        for (int i = 0; i < passes && !rc; ++i)
-            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, st) != hipSuccess) rc = -1;
+        for (int i = 0; i < passes && !rc; ++i, ++launched)
+            if (launch_grad_pass(precision, v, d_inst, d_tiles, P.ntiles, launched & 1, st) != hipSuccess) rc = -1;
         if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = -1;
-        Ctrl fin{};
-        if (!rc && hipMemcpy(&fin, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+        Ctrl fc[2];
+        if (!rc && hipMemcpy(fc, h.ctrl, sizeof fc, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+        const Ctrl fin = latest_ctrl(fc);
         if (rc) {
             set_error(__LINE__, __FILE__, __func__, "tuning pass failed on the device");
             break;
@@ -610,7 +614,7 @@ int model_grad_variant(int nx, int ny, int nz, int precision)
     const double t_def = predict(DEFAULT_GRAD_VARIANT);
     int best = DEFAULT_GRAD_VARIANT;
     double t_best = t_def;
-    for (int v = STREAM_BASE; v < GRAD_VARIANTS; ++v) {
+    for (int v = 0; v < GRAD_VARIANTS; ++v) {
         const double t = predict(v);
         if (t < t_best) { t_best = t; best = v; }
     }
@@ -660,8 +664,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     Tile *d_tiles = cv.take<Tile>((size_t)ntiles);
     Inst *d_inst = &d_head->inst;
     if ((char *)d_tiles != (char *)d_head + sizeof(SolveHead)) DT_FAIL("solve head layout");
-    Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, &d_head->ctrl,
-                       &d_head->counter);
+    Inst h = make_inst(P, dx, dy, dz, *p, d_target, d_mask, d_field, cv, trace, trace_cap, d_head->ctrl,
+                       d_head->acc);
     unsigned long long *status = nullptr;
     Ctrl *fin_word = nullptr;   // the control block, published by the device once done
     if (status_word(&status, &h.status, &fin_word, &h.final_ctrl)) return -1;
@@ -671,8 +675,8 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     // on this device has finished with the staging: solves end synchronised)
     SolveHead *sh = (SolveHead *)A->stage;
     sh->inst = h;
-    sh->ctrl = initial_ctrl(algorithm, *p);
-    sh->counter = 0;
+    sh->ctrl[0] = sh->ctrl[1] = initial_ctrl(algorithm, *p);
+    std::memset(sh->acc, 0, sizeof sh->acc);
     std::vector<Tile> tiles;
     tiles.reserve(ntiles);
     append_tiles_xcd(tiles, {{0, ntiles}});
@@ -693,22 +697,25 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     long long launched = 0;
     int rc = 0;
     auto enqueue = [&](int k) -> int {
-        for (int i = 0; i < k; ++i) {
+        for (int i = 0; i < k; ++i, ++launched) {
+            // a pass reads the control copy of its parity
+            const int parity = (int)(launched & 1);
             if (algorithm == DT_GRADIENT) {
-                DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, st));
+                DT_HIP(launch_grad_pass(precision, variant, d_inst, d_tiles, ntiles, parity, st));
             } else {
-                DT_HIP(launch_greedy_pass(precision, variant, d_inst, d_tiles, ntiles, st));
+                DT_HIP(launch_greedy_pass(precision, variant, d_inst, d_tiles, ntiles, parity, st));
             }
         }
-        launched += k;
         return 0;
     };
 
-    // Two chunks in flight; the host follows the completion word the
-    // controller stores in host memory and enqueues the next chunk once the
-    // older one has run, so nothing but pass kernels enters the stream.  Small
-    // chunks bound the passes that run idle after the solve has stopped (128^3
-    // gradient solve: 2 idle launches with 2-pass chunks, 4 with 4-pass ones).
+    // Two chunks in flight; the host follows the completion word that the
+    // start of each pass stores in host memory (the passes before it have run)
+    // and enqueues the next chunk once the older one has run, so nothing but
+    // pass kernels enters the stream.  Small chunks bound the passes that run
+    // idle after the solve has stopped.  The controller step of a pass's trial
+    // runs at the start of the next pass, which is always enqueued: the host
+    // stops only once the word reports the end.
     const int chunk = 2;
     rc = hipEventRecord(kev[0], st) == hipSuccess ? 0 : -1;
     if (!rc) rc = enqueue(chunk);
@@ -721,14 +728,14 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         if (w < 0) { set_error(__LINE__, __FILE__, __func__, "device error during solve"); rc = -1; break; }
         published = w == 0 && done;
         if (w > 0) {
-            // the stream drained but the word did not follow: read the control block
-            Ctrl c;
-            if (hipMemcpy(&c, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost) != hipSuccess) {
+            // the stream drained but the word did not follow: read the control copies
+            Ctrl c[2];
+            if (hipMemcpy(c, h.ctrl, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) {
                 set_error(__LINE__, __FILE__, __func__, "device error during solve");
                 rc = -1;
                 break;
             }
-            done = c.phase == PH_DONE;
+            done = latest_ctrl(c).phase == PH_DONE;
         }
         if (done) break;
         if (launched > pass_cap) {
@@ -742,10 +749,10 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
     // result into the caller's field behind the passes (and the final control
     // block, unless the device published it)
-    Ctrl *fin_h = published ? fin_word : (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
+    Ctrl *fin_copies = (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
     hipError_t se = rc ? hipSuccess : launch_take_result(d_inst, n, st);
     if (se == hipSuccess && !rc && !published)
-        se = hipMemcpyAsync(fin_h, h.ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st);
+        se = hipMemcpyAsync(fin_copies, h.ctrl, 2 * sizeof(Ctrl), hipMemcpyDeviceToHost, st);
     const hipError_t ss = hipStreamSynchronize(st);
     if (se == hipSuccess) se = ss;
     float kernel_ms = 0.0f;
@@ -754,7 +761,13 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     if (rc) return -1;
 
     Ctrl fin;
-    std::memcpy(&fin, fin_h, sizeof(Ctrl));
+    if (published) {
+        std::memcpy(&fin, fin_word, sizeof(Ctrl));
+    } else {
+        Ctrl c[2];
+        std::memcpy(c, fin_copies, sizeof c);
+        fin = latest_ctrl(c);
+    }
     if (trace && trace_cap > 0) {
         const long long nl = std::min(fin.trace_len, trace_cap);
         if (nl > 0) DT_HIP(hipMemcpy(trace, h.trace, (size_t)nl * 4 * sizeof(double), hipMemcpyDeviceToHost));

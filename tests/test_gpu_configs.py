@@ -6,8 +6,8 @@
   bit-identical to the reference C at one thread), an iteration count within
   10 % of the FP64 oracle's, and a final masked max |J - T| within 1e-4 of it.
 * The 64-volume batch (each axis U{48..160}): every engine result is
-  bit-identical to the lone dt_solve_device solve of the same volume (same
-  column tile, lone-solve z-chunks), and a seeded subset of 8 volumes --
+  bit-identical to the lone dt_solve_device solve of the same volume (each
+  with its own tiling: the sums are exact), and a seeded subset of 8 volumes --
   the two smallest, the two largest and four drawn at random -- holds the
   parity bar against the FP64 oracle.
 * A 30-request serving stream (Poisson arrivals, log-normal cube sides,
@@ -100,17 +100,16 @@ def _oracle_subset(sizes, k_random=4, seed=99):
     return pick
 
 
-def test_batch_64_engine_equals_lone_solves_and_oracle(gpu_lib, oracle32, oracle64, monkeypatch):
+def test_batch_64_engine_equals_lone_solves_and_oracle(gpu_lib, oracle32, oracle64):
     import torch
     from disptools_amd.workloads import batch_uneven
-    # one column tile on both sides: a lone solve picks its tile per shape,
-    # an engine lane uses one
-    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
+    # the engine's default tiling (one column tile per lane, deeper z-chunks)
+    # against the tiling each lone solve picks: the results are the same
     cases = batch_uneven(64, workers=8)
     print("batch of", len(cases), "volumes,", sum(c[0].size for c in cases), "voxels", flush=True)
     devs = [_DevVolume(T, M) for T, M, _ in cases]
     torch.cuda.synchronize()
-    with gpu_lib.Engine(tz=-1, max_slots=64) as eng:
+    with gpu_lib.Engine(max_slots=64) as eng:
         tickets = {}
         for i, (T, M, sp) in enumerate(cases):
             nz, ny, nx = T.shape
@@ -142,11 +141,9 @@ def test_batch_64_engine_equals_lone_solves_and_oracle(gpu_lib, oracle32, oracle
             _check_against_fp64(oracle32, oracle64, "gradient", T, M, sp, 2000, fields[i], res[i][1], "batch[%d]" % i)
 
 
-def test_serving_stream_30_requests(gpu_lib, oracle32, oracle64, monkeypatch):
+def test_serving_stream_30_requests(gpu_lib, oracle32, oracle64):
     from disptools_amd.serving import SolveRequest, SolveServer
     from disptools_amd.workloads import serving_stream, volume_case
-    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
-    monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", "1")
     stream = serving_stream(30, 80.0)
     reqs = []
     for k, r in enumerate(stream):
@@ -154,7 +151,7 @@ def test_serving_stream_30_requests(gpu_lib, oracle32, oracle64, monkeypatch):
         T, M = volume_case((side, side, side), r["seed"])
         reqs.append(SolveRequest(T, M, (1.0, 1.0, 1.0), r["algorithm"], {"it_max": 2000}, request_id=str(k)))
     print("30 requests, sides", [r["side"] for r in stream], flush=True)
-    with SolveServer(tz=-1) as srv:
+    with SolveServer() as srv:
         out = {r.request_id: r for r in srv.run(reqs)}
     assert sorted(out, key=int) == [r.request_id for r in reqs]
     for r in reqs:

@@ -116,15 +116,17 @@ def test_engine_alone_equals_mixed(gpu_lib):
         np.testing.assert_array_equal(mixed_f[i], alone_f[i])
 
 
-def test_engine_lone_tiling_equals_plain_solve(gpu_lib, monkeypatch):
-    # the same column tile on both sides (a single solve picks its tile per
-    # shape; an engine lane uses one)
-    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", "15")
+@pytest.mark.parametrize("tz", [-1, 0, 8])
+def test_engine_equals_plain_solve_for_any_tiling(gpu_lib, tz):
+    """The engine's result equals dt_solve_device's, bit for bit, whatever the
+    tiling: lone-solve z-chunks (-1), the batch default (0) or 8 planes, and
+    the lane's column tile against the one a single solve picks per shape
+    (the pass sums are exact)."""
     cases = _cases()
-    with gpu_lib.Engine(tz=-1) as eng:
+    with gpu_lib.Engine(tz=tz) as eng:
         ef, er = _run_all(eng, cases)
     for i, (alg, T, M, sp, it_max) in enumerate(cases):
-        print("plain", alg, T.shape, flush=True)
+        print("plain", alg, T.shape, "engine tz", tz, flush=True)
         f, info, _ = gpu_lib.solve_host(alg, T, M, sp, it_max=it_max)
         for k in SAME_KEYS:
             assert er[i][1][k] == info[k], (i, k, er[i][1][k], info[k])

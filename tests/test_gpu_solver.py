@@ -80,23 +80,31 @@ def test_solver_parity(gpu_lib, oracle32, oracle64, alg, case):
 _ORACLE64 = {}
 
 
-@pytest.mark.parametrize("variant", [str(v) for v in range(16)])
 @pytest.mark.parametrize("case", ["config1_40", "ragged_13x21x34"])
-def test_gradient_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant, case):
-    """Every compiled form and column tile of the gradient pass holds the
-    parity bar (a single solve picks its tile per shape)."""
+def test_gradient_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, case):
+    """Every column tile of the gradient pass gives the same solve, bit for bit
+    (the pass's sums are exact, so the tiling does not change them), and that
+    solve holds the parity bar (a single solve picks its tile per shape)."""
     T, M, sp, it_max = CASES[case]()
     if case not in _ORACLE64:
         _ORACLE64[case] = oracle64.solve("gradient", T.astype(np.float64), M, sp, it_max=it_max)
     fo, io = _ORACLE64[case]
-    monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", variant)
-    print("gradient variant", variant, case, T.shape, flush=True)
-    fg, ig, _ = gpu_lib.solve_host("gradient", T, M, sp, it_max=it_max)
-    assert ig["reason"] == io["reason"], (ig["reason"], io["reason"])
     _, mo = masked_max_error(oracle32, fo.astype(np.float32), T, M, sp)
-    _, mg = masked_max_error(oracle32, fg, T, M, sp)
-    assert abs(mg - mo) <= MAX_ERR_TOL
-    assert abs(ig["iterations"] - io["iterations"]) <= max(3, 0.1 * io["iterations"])
+    ref = None
+    for variant in range(5):
+        monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", str(variant))
+        print("gradient variant", variant, case, T.shape, flush=True)
+        fg, ig, _ = gpu_lib.solve_host("gradient", T, M, sp, it_max=it_max)
+        assert ig["reason"] == io["reason"], (ig["reason"], io["reason"])
+        _, mg = masked_max_error(oracle32, fg, T, M, sp)
+        assert abs(mg - mo) <= MAX_ERR_TOL
+        assert abs(ig["iterations"] - io["iterations"]) <= max(3, 0.1 * io["iterations"])
+        if ref is None:
+            ref = (fg, ig)
+            continue
+        np.testing.assert_array_equal(fg, ref[0])
+        for k in ("iterations", "evaluations", "rejections", "reason", "final_error", "final_max_error"):
+            assert ig[k] == ref[1][k], (variant, k)
 
 
 @pytest.mark.parametrize("alg", ["gradient", "greedy"])
@@ -238,12 +246,11 @@ def test_autotuner_picks_a_timed_variant_and_keeps_parity(gpu_lib, oracle32, ora
     assert abs(mg - mo) <= MAX_ERR_TOL
 
 
-@pytest.mark.parametrize("variant", ["0", "1", "2"])
+@pytest.mark.parametrize("variant", ["1", "2"])
 @pytest.mark.parametrize("precision", ["32", "64"])
 def test_greedy_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant, precision):
-    """The fused greedy pass on 32x16 (1) and 32x32 (2) tiles and the step +
-    evaluation kernels (0), FP32 and FP64 stencils, each to the parity bar of
-    test_solver_parity."""
+    """The fused greedy pass on 32x16 (1) and 32x32 (2) tiles, FP32 and FP64
+    stencils, each to the parity bar of test_solver_parity."""
     monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", variant)
     monkeypatch.setenv("DISPTOOLS_STENCIL_PRECISION", precision)
     T, M, sp, it_max = CASES["sphere24_aniso"]()
@@ -260,23 +267,19 @@ def test_greedy_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch, variant,
     np.testing.assert_allclose(trace[:k, 1], io["trace"][:k, 1], rtol=1e-4)
 
 
-def test_greedy_first_step_matches_between_variants(gpu_lib, monkeypatch):
-    """One greedy iteration: the fused pass forms the same trial as the step
-    kernel (the error maps it steps along agree to ~1 ulp of J)."""
+def test_greedy_tiles_give_the_same_solve(gpu_lib, monkeypatch):
+    """The two tilings of the greedy pass form and evaluate the same trials and
+    sum them exactly: the whole solve is the same, bit for bit."""
     from disptools_amd.workloads import volume_case
     T, M = volume_case((37, 29, 23), 3)
     sp = (1.0, 1.2, 0.9)
     out = {}
-    for v in ("0", "1", "2"):
-        monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", v)
-        print("greedy it_max=1 variant", v, flush=True)
-        out[v] = gpu_lib.solve_host("greedy", T, M, sp, it_max=1)
-    f0, i0, _ = out["0"]
     for v in ("1", "2"):
-        f1, i1, _ = out[v]
-        assert i0["reason"] == i1["reason"] and i0["iterations"] == i1["iterations"]
-        np.testing.assert_allclose(f1, f0, rtol=0, atol=2e-6)
-        assert i1["final_error"] == pytest.approx(i0["final_error"], rel=1e-5)
-    # the two fused tilings group the FP64 partial sums differently but form
-    # and evaluate the same trial
-    assert np.array_equal(out["1"][0], out["2"][0])
+        monkeypatch.setenv("DISPTOOLS_GREEDY_VARIANT", v)
+        print("greedy variant", v, flush=True)
+        out[v] = gpu_lib.solve_host("greedy", T, M, sp, it_max=200)
+    f1, i1, _ = out["1"]
+    f2, i2, _ = out["2"]
+    np.testing.assert_array_equal(f1, f2)
+    for k in ("iterations", "evaluations", "rejections", "reason", "final_error", "final_max_error"):
+        assert i1[k] == i2[k], k
