@@ -95,10 +95,9 @@ __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoe
                                                 : jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
-    if (count) {
-        Egrp += (double)e * (double)e;
-        if (in_mask) mx = fmaxf(mx, fabsf(e));
-    }
+    const float ec = count ? e : 0.0f;
+    Egrp += (double)ec * (double)ec;
+    mx = (count && in_mask) ? fmaxf(mx, fabsf(e)) : mx;
     const CT cw = (CT)penalised_weight(e, J, P);
     CT d2[3][3];
 #pragma unroll
@@ -153,21 +152,22 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const int z0 = tc.z0, z1 = tc.z1;
     const unsigned cn4 = 4u * n;   // bytes between components
 
-    // primary point of this thread: a tile column (tid < NOWN) or a ring point
+    // primary point of this thread: a tile column (tid < NOWN) or a ring point.
+    // Tile threads fill whole waves, so `own` is uniform over a wave.  The
+    // ring waves' spare lanes repeat ring points (the same values land at the
+    // same places), so every lane has a point and no lane's work is skipped
+    // by a branch.
     const int tid = threadIdx.x;
-    const bool own = tid < NOWN;
-    const int rr = tid - NOWN;
-    const bool ring = !own && rr < NRING;
-    int lx = 0, ly = 0, what = 0;   // what: 0 tile, 1 x-ring (W_{a,x}), 2 y-ring (W_{a,y}), 3 none
+    const bool own = (int)uni((unsigned)tid) < NOWN;
+    const int rr = (tid - NOWN) % NRING;
+    int lx = 0, ly = 0, what = 0;   // what: 0 tile, 1 x-ring (W_{a,x}), 2 y-ring (W_{a,y})
     if (own) {
         lx = tid % TX; ly = tid / TX;
-    } else if (ring) {
+    } else {
         if (rr < TY) { lx = -1; ly = rr; what = 1; }
         else if (rr < 2 * TY) { lx = TX; ly = rr - TY; what = 1; }
         else if (rr < 2 * TY + TX) { lx = rr - 2 * TY; ly = -1; what = 2; }
         else { lx = rr - 2 * TY - TX; ly = TY; what = 2; }
-    } else {
-        what = 3;
     }
     const int gx = tc.x0 + lx, gy = tc.y0 + ly;
     if (own) { S.accE[tid] = 0; S.accG[tid] = 0; }
@@ -181,29 +181,27 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         }
         grp = 0.0;
     };
-    const bool pin = what != 3 && gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
+    const bool pin = gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
     const bool col_interior = own && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
     const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
     const ColCoef K = column_coefficients(P, gx, gy);
-    const int pi = what == 3 ? -1 : (ly + 3) * HX + (lx + 3);
+    const int pi = (ly + 3) * HX + (lx + 3);
 
-    // halo slots: a share of rings 2-3 and the ring-1 corners
+    // halo slots: a share of rings 2-3 and the ring-1 corners (slots past the
+    // last point repeat points from the start)
     unsigned xv[SX];   // voffset
-    int xd[SX];        // LDS index in a plane triple, -1: none
+    int xd[SX];        // LDS index in a plane triple
     unsigned xok = 0;
 #pragma unroll
     for (int j = 0; j < SX; ++j) {
-        const int o = tid + j * NT;
-        const bool v = o < 3 * NOTHER;
-        int c = 0, xx = 0, yy = 0;
-        if (v) {
-            c = o / NOTHER;
-            other_point<TX, TY>(o - c * NOTHER, xx, yy);
-        }
+        const int o = (tid + j * NT) % (3 * NOTHER);
+        const int c = o / NOTHER;
+        int xx = 0, yy = 0;
+        other_point<TX, TY>(o - c * NOTHER, xx, yy);
         const int x = tc.x0 - 3 + xx, y = tc.y0 - 3 + yy;
-        const bool in = v && x >= 0 && x < nx && y >= 0 && y < ny;
+        const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
         xv[j] = in ? 4u * ((unsigned)c * n + (unsigned)(y * nx + x)) : 0u;
-        xd[j] = v ? c * PL + yy * HX + xx : -1;
+        xd[j] = c * PL + yy * HX + xx;
         xok |= (in ? 1u : 0u) << j;
     }
 
@@ -238,13 +236,10 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     };
     auto commit = [&](int slot, const Val &o) {
         float *dst = &S.xs[slot][0][0];
-        if (pi >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) dst[c * PL + pi] = o.v[c];
-        }
+        for (int c = 0; c < 3; ++c) dst[c * PL + pi] = o.v[c];
 #pragma unroll
-        for (int j = 0; j < SX; ++j)
-            if (xd[j] >= 0) dst[xd[j]] = o.v[3 + j];
+        for (int j = 0; j < SX; ++j) dst[xd[j]] = o.v[3 + j];
     };
     struct TM { float t; unsigned m; };
     auto fetch_tm = [&](int gz, TM &o) {
@@ -343,9 +338,10 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         fetch(zz + 3, rn);
         fetch_tm(zz + 1, nxt);
 
-        // (A) plane zz at the primary point
+        // (A) plane zz at the primary point (computed at points outside the
+        // volume too: nothing reads their W, their error is not counted)
         CT wz_p[3] = {0, 0, 0};
-        if (zz >= 0 && zz < nz && pin) {
+        if (zz >= 0 && zz < nz) {
             const float *R0 = &S.xs[sa][0][0] + pi;
             float sm[3][3][4];
 #pragma unroll
@@ -355,17 +351,21 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                 sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
                 sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
             }
-            const bool count = own && zz >= z0 && zz < z1;
+            const bool count = own && pin && zz >= z0 && zz < z1;
             CT W[9];
             stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
             const int pz = zz & 1;
-            if (what != 2) {
+            if (own) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) S.wx[pz][a][ly][lx + 1] = (float)W[a];
-            }
-            if (what != 1) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) S.wy[pz][a][ly + 1][lx] = (float)W[3 + a];
+            } else {
+                // a ring point stores its relation axis' three values
+                float *wdst = what == 1 ? &S.wx[pz][0][ly][lx + 1] : &S.wy[pz][0][ly + 1][lx];
+                const int wstride = what == 1 ? TY * (TX + 2) : (TY + 2) * TX;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) wdst[a * wstride] = (float)(what == 1 ? W[a] : W[3 + a]);
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
@@ -375,25 +375,26 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
-        if (own && pin && zb >= z0 && zb < z1) {
+        if (own && zb >= z0 && zb < z1) {
             const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-            if (col_interior && zb >= 1 && zb <= nz - 2) {
+            if (zb >= 1 && zb <= nz - 2) {
                 const CT dxx = (CT)S.wx[pb][0][ly][lx] - (CT)S.wx[pb][0][ly][lx + 2];
                 const CT dyx = (CT)S.wx[pb][1][ly][lx] - (CT)S.wx[pb][1][ly][lx + 2];
                 const CT dzx = (CT)S.wx[pb][2][ly][lx] - (CT)S.wx[pb][2][ly][lx + 2];
                 const CT dxy = (CT)S.wy[pb][0][ly][lx] - (CT)S.wy[pb][0][ly + 2][lx];
                 const CT dyy = (CT)S.wy[pb][1][ly][lx] - (CT)S.wy[pb][1][ly + 2][lx];
                 const CT dzy = (CT)S.wy[pb][2][ly][lx] - (CT)S.wy[pb][2][ly + 2][lx];
-                f0 = (float)(cix * dxx - ciy * dxy + ciz * (wz_m[0] - wz_p[0]));
-                f1 = (float)(-cix * dyx + ciy * dyy - ciz * (wz_m[1] - wz_p[1]));
-                f2 = (float)(cix * dzx - ciy * dzy + ciz * (wz_m[2] - wz_p[2]));
+                // the gradient is zero on the outer face (and unused outside the volume)
+                f0 = col_interior ? (float)(cix * dxx - ciy * dxy + ciz * (wz_m[0] - wz_p[0])) : 0.0f;
+                f1 = col_interior ? (float)(-cix * dyx + ciy * dyy - ciz * (wz_m[1] - wz_p[1])) : 0.0f;
+                f2 = col_interior ? (float)(cix * dzx - ciy * dzy + ciz * (wz_m[2] - wz_p[2])) : 0.0f;
                 Ggrp += (double)f0 * f0 + (double)f1 * f1 + (double)f2 * f2;
             }
             // trial value of this voxel on plane zb: window slot 1 (planes zz-2 .. zz+2);
             // stored at the start of the next step
             const float ux = col[1][0], uy = col[1][1], uz = col[1][2];
-            pend.on = true;
+            pend.on = pin;
             pend.n[0] = ux - eta_next * f0; pend.n[1] = uy - eta_next * f1; pend.n[2] = uz - eta_next * f2;
         }
         if (zb >= z0 && zb < z1 && ((zb & 3) == 3 || zb == nz - 1)) flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
