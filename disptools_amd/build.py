@@ -40,6 +40,7 @@ HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_matching.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_pyramid.hip", "-ffp-contract=off"),
+               ("dt_noalloc.hip", "-ffp-contract=off"),
                ("dt_runtime.cpp", "-ffp-contract=off"),
                ("dt_engine.cpp", "-ffp-contract=off")]
 

@@ -113,6 +113,9 @@ lib.dt_solve_device.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _v
                                 ctypes.POINTER(Params), _vp, ctypes.POINTER(SolveInfo), _vp,
                                 ctypes.c_longlong, _vp]
 lib.dt_solve_device.restype = ctypes.c_int
+lib.dt_solve_device_noalloc.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp,
+                                        ctypes.POINTER(Params), _vp, ctypes.POINTER(SolveInfo), _vp]
+lib.dt_solve_device_noalloc.restype = ctypes.c_int
 lib.dt_solve_host.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp,
                               ctypes.POINTER(Params), _vp, ctypes.POINTER(SolveInfo), _vp,
                               ctypes.c_longlong]
@@ -303,6 +306,17 @@ def solve_device(algorithm, nx, ny, nz, spacing, d_target, d_mask, d_field, stre
     if rc != 0:
         raise HipError(last_error())
     return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def solve_device_noalloc(algorithm, nx, ny, nz, spacing, d_target, d_mask, d_field, stream=0, **params):
+    """dt_solve_device_noalloc on device pointers (ints); returns the info dict."""
+    info = SolveInfo()
+    p = make_params(**params)
+    rc = lib.dt_solve_device_noalloc(ALGORITHMS[algorithm], nx, ny, nz, *map(float, spacing), d_target, d_mask,
+                                     ctypes.byref(p), d_field, ctypes.byref(info), stream or None)
+    if rc != 0:
+        raise HipError(last_error())
+    return info.as_dict()
 
 
 def matching_host(target, spacing=(1.0, 1.0, 1.0), field=None, it_max=50000, trace_cap=0, epsilon=9.99e-4,

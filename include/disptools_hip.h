@@ -206,6 +206,28 @@ int dt_solve_device(int algorithm, size_t nx, size_t ny, size_t nz,
                     const dt_params *params, float *d_field,
                     dt_solve_info *info, double *trace, long long trace_cap, void *stream);
 
+/* Size of the device array that dt_solve_device_noalloc keeps its solver
+ * state in (80 MiB): a gradient solve of up to ~2.2 M voxels (128^3), a greedy
+ * solve of up to ~4.1 M voxels (161^3).  Each GPU the library runs on holds
+ * one such array until the process ends. */
+#define DT_NOALLOC_SCRATCH_BYTES (80u * 1024u * 1024u)
+
+/* No counterpart in the reference: dt_solve_device for callers that cannot
+ * allocate memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same arguments and results as
+ * dt_solve_device without the trace; calls no hipMalloc, hipFree,
+ * hipHostMalloc or event creation: the solver's state lives in a static
+ * device array of DT_NOALLOC_SCRATCH_BYTES (one per GPU), set up by a kernel.
+ * A volume whose state does not fit is rejected (-1, error state set) before
+ * any device work.  Calls on one GPU share its array and run one at a time
+ * within the library; calls from outside it that overlap on one GPU race.
+ * info->seconds and info->kernel_seconds are 0 (no events are created). */
+int dt_solve_device_noalloc(int algorithm, size_t nx, size_t ny, size_t nz,
+                            float dx, float dy, float dz,
+                            const float *d_target, const unsigned char *d_mask,
+                            const dt_params *params, float *d_field,
+                            dt_solve_info *info, void *stream);
+
 /* Same as dt_solve_device with host pointers (one H2D and one D2H copy). */
 int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
                   float dx, float dy, float dz,
