@@ -96,7 +96,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy", "pyramid"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy", "pyramid", "jacobian"), default="solve")
     ap.add_argument("--algorithm", choices=("gradient", "greedy", "matching"), default="gradient",
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
@@ -110,6 +110,8 @@ def parse():
     ap.add_argument("--qps", type=float, default=80.0)
     ap.add_argument("--graphs", type=int, default=0, help="engine replays chunks from hipGraphs")
     ap.add_argument("--slots", type=int, default=0, help="engine slots (instances solved at once; 0: 256)")
+    ap.add_argument("--tz", type=int, default=0,
+                    help="engine z-chunk policy (0: re-planned per set, -1: lone-solve, -2: <= 16, > 0: fixed)")
     ap.add_argument("--serve-dump", default="", help="write per-request records of the serving run (JSON)")
     return ap.parse_args()
 
@@ -169,7 +171,7 @@ def _iterations(info):
     return info["iterations"] - (1 if info["reason"] == 6 else 0)
 
 
-def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160)):
+def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160), tz=0):
     """64 volumes of uneven sizes (each axis U{48..160}, spacing 1): solved one
     dt_solve_device (matching: dt_matching_device) call after the other (the
     plain one-volume-per-launch solve) and all at once by the batch engine,
@@ -209,7 +211,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160)):
     vols[big][2].zero_()
     vols[0][2].zero_()
     plain(0)
-    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz) as eng:
         vols[0][2].zero_()
         T, M, F, sp = vols[0]
         eng.submit(algorithm, T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
@@ -231,7 +233,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160)):
     for v in vols:
         v[2].zero_()
     torch.cuda.synchronize()
-    with native.Engine(use_graphs=bool(graphs), stream=sh) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz) as eng:
         t0 = time.perf_counter()
         tickets = {}
         for i, (T, M, F, sp) in enumerate(vols):
@@ -277,7 +279,7 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0.0):
+def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0.0, tz=0):
     """Open-loop serving stream: Poisson arrivals at `qps`, cube side
     log-normal (median 96, sigma_ln 0.35, clipped to [32, 256]), 80 % gradient
     and 20 % greedy requests, targets and masks of the 128^3 workload's
@@ -362,7 +364,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0
     vi_b = 0
     for d in data:
         d[2].zero_()
-    with native.Engine(use_graphs=bool(graphs), stream=sh, max_slots=slots) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, max_slots=slots, tz=tz) as eng:
         queue = collections.deque()
         tickets = {}
         nxt = 0
@@ -445,6 +447,38 @@ def pyramid_bench(it_max, reps=3):
     return out
 
 
+def jacobian_bench(sides=(128, 256), reps=50):
+    """dt_jacobian_device (k_jacobian) on a random field of side^3 voxels,
+    resident in HBM: average launch duration from HIP events on the launch
+    stream, against 16 algorithmic bytes per voxel (3 field floats read, J
+    written)."""
+    import torch
+    from disptools_amd import native
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    out = {"workload": "jacobian of a random field (N(0, 0.1) per component), spacing (1, 1, 1.5)",
+           "bytes_per_voxel": 16}
+    for side in sides:
+        g = torch.Generator(device=dev).manual_seed(side)
+        F = torch.randn((3, side, side, side), generator=g, device=dev, dtype=torch.float32) * 0.1
+        J = torch.empty((side, side, side), device=dev, dtype=torch.float32)
+        sp = (1.0, 1.0, 1.5)
+        for _ in range(3):
+            native.jacobian_device(side, side, side, sp, F.data_ptr(), J.data_ptr(), stream=stream.cuda_stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            native.jacobian_device(side, side, side, sp, F.data_ptr(), J.data_ptr(), stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        gbs = 16.0 * side ** 3 / (us * 1e-6) / 1e9
+        out["%d^3" % side] = {"avg_launch_us": us, "achieved_GBps": gbs, "frac": gbs / PEAK_HBM_GBS}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -457,17 +491,23 @@ def main():
     if args.mode == "pyramid":
         print(json.dumps({"mode": "pyramid", **pyramid_bench(args.it_max)}))
         return
+    if args.mode == "jacobian":
+        from disptools_amd import native
+        if not native.set_device(0):
+            raise RuntimeError("no HIP device")
+        print(json.dumps({"mode": "jacobian", **jacobian_bench()}))
+        return
     if args.mode in ("batch", "serve"):
         from disptools_amd import native
         if not native.set_device(0):
             raise RuntimeError("no HIP device")
         if args.mode == "batch":
             sides = tuple(int(v) for v in args.batch_sides.split(","))
-            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides)))
+            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides, args.tz)))
         else:
             print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
                                          matching=args.serve_matching,
-                                         slots=args.slots, dump=args.serve_dump)))
+                                         slots=args.slots, dump=args.serve_dump, tz=args.tz)))
         return
 
     import numpy as np
@@ -541,9 +581,9 @@ def main():
     avg_launch_s = sum(i["kernel_seconds"] for i in infos) / launches
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
-    batch = batch_bench(args.batch, args.it_max, args.graphs) if args.batch > 0 else None
+    batch = batch_bench(args.batch, args.it_max, args.graphs, tz=args.tz) if args.batch > 0 else None
     serving = (serve_bench(args.serve, args.qps, args.it_max, args.graphs, slots=args.slots,
-                           dump=args.serve_dump)
+                           dump=args.serve_dump, tz=args.tz)
                if args.serve > 0 else None)
 
     cores, cores_source = physical_cores()

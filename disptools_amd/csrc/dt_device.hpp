@@ -479,18 +479,17 @@ struct TileCoord {
 };
 
 template <int TX, int TY>
-__device__ __forceinline__ TileCoord decode_tile(const Inst &I, int t)
+__device__ __forceinline__ TileCoord decode_tile(const Inst &I, const Tile &t)
 {
     TileCoord c;
-    const int tx = t % I.ntx;
-    const int r = t / I.ntx;
-    const int ty = r % I.nty;
-    const int tz = r / I.nty;
-    c.x0 = tx * TX;
-    c.y0 = ty * TY;
-    c.z0 = tz * I.tz;
-    c.z1 = min(c.z0 + I.tz, I.nz);
+    c.x0 = t.tile % I.ntx * TX;
+    c.y0 = t.tile / I.ntx * TY;
+    c.z0 = t.z0;
+    c.z1 = t.z1;
     return c;
 }
+
+// The instance's first tile: the one that stores the next control block.
+__device__ __forceinline__ bool leader_tile(const Tile &t) { return t.tile == 0 && t.z0 == 0; }
 
 }  // namespace dt

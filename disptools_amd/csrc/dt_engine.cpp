@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -212,14 +213,63 @@ int slot_tiles(const dt_engine *e, int s)
     return sl.lane == DT_MATCHING ? e->h_minsts[s].nblocks : sl.P.ntiles;
 }
 
+// Depth of balanced z-chunks of at most `most` planes (a multiple of 4), in
+// whole groups of four planes.
+int balanced_tz(int nz, int most)
+{
+    const int groups = (nz + 3) / 4;
+    const int chunks = (groups + most / 4 - 1) / (most / 4);
+    return std::min((groups + chunks - 1) / chunks * 4, (nz + 3) / 4 * 4);
+}
+
+// Depth at admission: fixed (tz_policy > 0), as a lone solve (-1), else
+// balanced chunks of at most 16 planes (re-planned with the lane, below).
 int engine_tz(const dt_engine *e, int nz)
 {
     if (e->tz_policy > 0) return std::min(e->tz_policy, nz);
-    if (e->tz_policy < 0) return 0;   // make_plan: as a lone solve
-    // balanced chunks of at most 16 planes, in whole groups of four
-    const int groups = (nz + 3) / 4;
-    const int chunks = (groups + 3) / 4;
-    return (groups + chunks - 1) / chunks * 4;
+    if (e->tz_policy == -1) return 0;   // make_plan: as a lone solve
+    return balanced_tz(nz, 16);
+}
+
+// Re-plans the z-chunk depth of every instance of a gradient or greedy lane
+// (tz_policy 0) for the pass over the lane's current set: every instance is
+// cut into balanced chunks of at most D planes, D chosen to minimise the
+// estimated pass time.  A block of depth d costs d + O plane steps (O: the
+// halo planes of the pipeline and the block's fixed work -- control step,
+// sums -- in plane-step units); the CUs run the blocks in rounds, so a pass
+// of blocks of total cost W and largest cost c takes about
+// ceil(W / (CUs * c)) * c.  Small D fills the CUs when few instances remain,
+// large D saves halo planes when many do.  The results do not depend on the
+// depths (exact sums), so an instance may be re-tiled between any two passes.
+void retile_lane(dt_engine *e, Lane &L)
+{
+    const int cus = device_cus();
+    const int O = L.kind == DT_GREEDY ? 7 : 6;
+    int best_d = 16;
+    double best_cost = 1e300;
+    for (int D = 4; D <= 64; D += 4) {
+        double work = 0.0;
+        int cmax = 0;
+        for (int s = 0; s < (int)e->slots.size(); ++s) {
+            const Slot &sl = e->slots[s];
+            if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
+            const int d = balanced_tz(sl.P.nz, D);
+            const long long tiles = (long long)sl.P.ntx * sl.P.nty * ((sl.P.nz + d - 1) / d);
+            const int c = std::min(d, sl.P.nz) + O;
+            work += (double)tiles * c;
+            cmax = std::max(cmax, c);
+        }
+        if (cmax == 0) return;
+        const double cost = std::ceil(work / ((double)cus * cmax)) * cmax;
+        if (cost < best_cost * 0.999) { best_cost = cost; best_d = D; }
+    }
+    for (int s = 0; s < (int)e->slots.size(); ++s) {
+        Slot &sl = e->slots[s];
+        if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
+        sl.P.tz = balanced_tz(sl.P.nz, best_d);
+        sl.P.ntz = (sl.P.nz + sl.P.tz - 1) / sl.P.tz;
+        sl.P.ntiles = sl.P.ntx * sl.P.nty * sl.P.ntz;
+    }
 }
 
 Plan engine_plan(const dt_engine *e, int algorithm, int nx, int ny, int nz)
@@ -276,13 +326,14 @@ int ensure_tables(Lane &L, size_t need)
 // Rebuilds and uploads the lane's tile table through staging buffer `stage`.
 int rebuild_tables(dt_engine *e, Lane &L, int stage)
 {
-    std::vector<std::pair<int, int>> seg;
+    if (e->tz_policy == 0 && L.kind != DT_MATCHING) retile_lane(e, L);
+    std::vector<TileRun> seg;
     size_t count = 0;
     for (int s = 0; s < (int)e->slots.size(); ++s) {
         const Slot &sl = e->slots[s];
         if (sl.state != SLOT_ACTIVE || sl.lane != L.kind) continue;
         const int nt = slot_tiles(e, s);
-        seg.push_back({s, nt});
+        seg.push_back(sl.lane == DT_MATCHING ? TileRun{s, nt, 1, 1} : plan_run(s, sl.P));
         count += (size_t)nt;
     }
     if (ensure_tables(L, std::max<size_t>(count, 1))) return -1;

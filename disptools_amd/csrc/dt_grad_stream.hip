@@ -434,11 +434,11 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, in
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, tl.tile == 0, &S.ctrl, S.acc_w);
+    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl, S.acc_w);
     __syncthreads();
     const Ctrl *C = &S.ctrl;
     if (!uni(C->active)) return;
-    TileCoord tc = decode_tile<TX, TY>(I, tl.tile);
+    TileCoord tc = decode_tile<TX, TY>(I, tl);
     tc.x0 = uni(tc.x0); tc.y0 = uni(tc.y0); tc.z0 = uni(tc.z0); tc.z1 = uni(tc.z1);
 
     unsigned flags = 0;

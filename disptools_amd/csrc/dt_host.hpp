@@ -144,11 +144,18 @@ Ctrl initial_ctrl(int algorithm, const dt_params &p);
 // of the solve, or the one with more evaluated passes).
 const Ctrl &latest_ctrl(const Ctrl (&c)[2]);
 
-// Appends the tiles of `segments` (instance index, tile count) to `out`, in
-// the order that gives block b (dispatched to XCD b % 8) a contiguous run of
-// the concatenated tile list, so the halos a block re-reads sit in its own
-// XCD's L2.
-void append_tiles_xcd(std::vector<Tile> &out, const std::vector<std::pair<int, int>> &segments);
+// The tiles of one instance: `cols` column tiles times the z-chunks of depth
+// `tz` over `nz` planes (matching: cols blocks, nz = tz = 1).
+struct TileRun {
+    int inst, cols, nz, tz;
+};
+inline TileRun plan_run(int inst, const Plan &P) { return TileRun{inst, P.ntx * P.nty, P.nz, P.tz}; }
+
+// Appends the tiles of `runs` to `out` (per run: columns fastest, then
+// z-chunks), in the order that gives block b (dispatched to XCD b % 8) a
+// contiguous run of the concatenated tile list, so the halos a block re-reads
+// sit in its own XCD's L2.
+void append_tiles_xcd(std::vector<Tile> &out, const std::vector<TileRun> &runs);
 
 // Checks the arguments of a solve (sets the error state and returns -1).
 int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const dt_params *p);

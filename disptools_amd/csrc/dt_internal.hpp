@@ -118,10 +118,14 @@ struct Inst {
     Ctrl *final_ctrl;             // host-coherent copy of the control block once done (null: none)
 };
 
-// One entry of a launch's tile table.
+// One entry of a launch's tile table: a column tile of an instance and the
+// planes [z0, z1) the block covers (z0 a multiple of 4).  The tiling of an
+// instance is only in its table entries, so a table may re-tile an instance
+// from one pass to the next.
 struct Tile {
     int inst;   // instance index
-    int tile;   // tile index inside the instance
+    int tile;   // column tile tx + ntx * ty (matching: block index)
+    int z0, z1;
 };
 
 }  // namespace dt

@@ -74,6 +74,7 @@ k_noalloc_setup(NoAllocHead *head, Tile *tiles, Inst h, Ctrl c0, int ntiles, flo
     // valid pairs before it.
     constexpr int XCDS = 8;
     const int per = (ntiles + XCDS - 1) / XCDS;
+    const int cols = h.ntx * h.nty;
     for (long long t = g; t < (long long)ntiles; t += step) {
         const int x = (int)(t / per), k = (int)(t % per);
         long long pos = 0;
@@ -82,7 +83,8 @@ k_noalloc_setup(NoAllocHead *head, Tile *tiles, Inst h, Ctrl c0, int ntiles, flo
             const int cnt = min(max(ntiles - xx * per, 0), per);   // tiles of XCD xx
             pos += min(cnt, k) + ((xx < x && cnt > k) ? 1 : 0);
         }
-        tiles[pos] = Tile{0, (int)t};
+        const int z0 = (int)(t / cols) * h.tz;
+        tiles[pos] = Tile{0, (int)(t % cols), z0, min(z0 + h.tz, h.nz)};
     }
     for (long long i = g; i < nclear; i += step) clear[i] = 0.0f;
 }

@@ -434,12 +434,13 @@ Plan make_plan(int algorithm, int nx, int ny, int nz, int precision, int variant
     return P;
 }
 
-void append_tiles_xcd(std::vector<Tile> &out, const std::vector<std::pair<int, int>> &segments)
+void append_tiles_xcd(std::vector<Tile> &out, const std::vector<TileRun> &runs)
 {
     constexpr int XCDS = 8;
     std::vector<Tile> flat;
-    for (const auto &s : segments)
-        for (int t = 0; t < s.second; ++t) flat.push_back(Tile{s.first, t});
+    for (const TileRun &r : runs)
+        for (int z0 = 0; z0 < r.nz; z0 += r.tz)
+            for (int c = 0; c < r.cols; ++c) flat.push_back(Tile{r.inst, c, z0, std::min(z0 + r.tz, r.nz)});
     const size_t total = flat.size();
     const size_t per = (total + XCDS - 1) / XCDS;
     for (size_t k = 0; k < per; ++k)
@@ -532,7 +533,7 @@ static int tune_gradient(int nx, int ny, int nz, float dx, float dy, float dz, i
         Inst *d_inst = cv.take<Inst>(1);
         Tile *d_tiles = cv.take<Tile>((size_t)P.ntiles);
         std::vector<Tile> tiles;
-        append_tiles_xcd(tiles, {{0, P.ntiles}});
+        append_tiles_xcd(tiles, {plan_run(0, P)});
         const Ctrl c0[2] = {initial_ctrl(DT_GRADIENT, p), initial_ctrl(DT_GRADIENT, p)};
         float ms = 0.0f;
         if (hipMemsetAsync(d_field, 0, 12 * (size_t)n, st) != hipSuccess ||
@@ -679,7 +680,7 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
     std::memset(sh->acc, 0, sizeof sh->acc);
     std::vector<Tile> tiles;
     tiles.reserve(ntiles);
-    append_tiles_xcd(tiles, {{0, ntiles}});
+    append_tiles_xcd(tiles, {plan_run(0, P)});
     std::memcpy(A->stage + sizeof(SolveHead), tiles.data(), tiles.size() * sizeof(Tile));
     DT_HIP(hipMemcpyAsync(d_head, A->stage, head, hipMemcpyHostToDevice, st));
     if (algorithm == DT_GREEDY) {

@@ -287,8 +287,11 @@ typedef struct dt_engine_config {
     int chunk;           /* passes enqueued per chunk (0: 2)                               */
     int use_graphs;      /* 1: each chunk is replayed from a captured hipGraph             */
     int variant;         /* gradient pass variant (-1: the library default)                */
-    int tz;              /* tile depth in planes: 0 = batch default (balanced, <= 16),
-                            -1 = as a lone dt_solve_device call, > 0 = fixed              */
+    int tz;              /* z-chunk depth in planes: 0 = re-planned for the set of
+                            instances whenever it changes (balanced chunks, the bound
+                            that fills the CUs best), -1 = as a lone dt_solve_device
+                            call, -2 = balanced chunks of at most 16 planes, > 0 =
+                            fixed; results are the same for every choice            */
 } dt_engine_config;
 
 typedef struct dt_engine_result {

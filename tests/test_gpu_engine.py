@@ -116,10 +116,11 @@ def test_engine_alone_equals_mixed(gpu_lib):
         np.testing.assert_array_equal(mixed_f[i], alone_f[i])
 
 
-@pytest.mark.parametrize("tz", [-1, 0, 8])
+@pytest.mark.parametrize("tz", [-1, 0, -2, 8])
 def test_engine_equals_plain_solve_for_any_tiling(gpu_lib, tz):
     """The engine's result equals dt_solve_device's, bit for bit, whatever the
-    tiling: lone-solve z-chunks (-1), the batch default (0) or 8 planes, and
+    tiling: lone-solve z-chunks (-1), the batch default (0: re-planned as the
+    set of instances changes), chunks of at most 16 planes (-2) or 8 planes, and
     the lane's column tile against the one a single solve picks per shape
     (the pass sums are exact)."""
     cases = _cases()
