@@ -37,6 +37,7 @@ HIP_FLAGS = [
 # standalone jacobian keeps every multiply and add separately rounded (it is
 # bit-exact with the reference); the solver pass is FP32 with FMA contraction.
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
+               ("dt_jacobian.hip", "-ffp-contract=off"),
                ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_matching.hip", "-ffp-contract=fast-honor-pragmas"),
                ("dt_pyramid.hip", "-ffp-contract=off"),

@@ -1,7 +1,7 @@
 // dt_kernels.hip -- gfx950 kernels of the displacement-generation path
-// besides the solver passes (dt_grad_stream.hip, dt_greedy_stream.hip):
+// besides the solver passes (dt_grad_stream.hip, dt_greedy_stream.hip) and
+// the standalone Jacobian (dt_jacobian.hip):
 //
-//   k_jacobian     det(I + grad u) over a volume (reference jacobian, src/jacobian.c:70-242)
 //   k_regularise   J = max(J, epsilon) (src/jacobian.c:29-55)
 //   k_take_result  the solve's result into the caller's field buffer
 #include <math.h>
@@ -11,51 +11,9 @@
 
 namespace dt {
 
-// Jacobian at one voxel read straight from global memory (region map of src/jacobian.c).
-template <typename CT>
-__device__ __forceinline__ float jacobian_global(const float *__restrict__ f, long long n, int x, int y, int z,
-                                                 int nx, int ny, int nz, float ix, float iy, float iz)
-{
-    int kind[3];
-    classify(x, y, z, nx, ny, nz, kind);
-    const long long i = ((long long)z * ny + y) * nx + x;
-    const long long st[3] = {1, nx, (long long)nx * ny};
-    const float inv[3] = {ix, iy, iz};
-    CT d[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float *p = f + c * n + i;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const long long s = st[a];
-            const int k = kind[a];
-            const float v0 = p[0];
-            const float m1 = (k == K_C4 || k == K_C2 || k == K_BWD) ? p[-s] : 0.0f;
-            const float p1 = (k == K_C4 || k == K_C2 || k == K_FWD) ? p[s] : 0.0f;
-            const float m2 = (k == K_C4 || k == K_BWD) ? p[-2 * s] : 0.0f;
-            const float p2 = (k == K_C4 || k == K_FWD) ? p[2 * s] : 0.0f;
-            d[c][a] = dstencil<CT>(k, m2, m1, v0, p1, p2, inv[a]);
-        }
-    }
-    return (float)det_plus_identity<CT>(d);
-}
-
 // ---------------------------------------------------------------------------
-// Standalone Jacobian and regularisation
+// Regularisation and result copy
 // ---------------------------------------------------------------------------
-
-template <typename CT>
-__global__ void __launch_bounds__(256)
-k_jacobian(const float *__restrict__ f, float *__restrict__ J, int nx, int ny, int nz,
-           float ix, float iy, float iz)
-{
-    const long long n = (long long)nx * ny * nz;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
-    if (x >= nx || y >= ny) return;
-    J[((long long)z * ny + y) * nx + x] = jacobian_global<CT>(f, n, x, y, z, nx, ny, nz, ix, iy, iz);
-}
 
 __global__ void __launch_bounds__(256)
 k_regularise(float *__restrict__ J, long long n, float eps)
@@ -100,14 +58,6 @@ hipError_t launch_take_result(const Inst *inst, long long n, hipStream_t st)
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_take_result, dim3((unsigned)blocks), dim3(256), 0, st, inst);
-    return hipGetLastError();
-}
-
-hipError_t launch_jacobian(const float *f, float *J, int nx, int ny, int nz, float ix, float iy, float iz,
-                           hipStream_t st)
-{
-    dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
-    hipLaunchKernelGGL(k_jacobian<double>, grid, dim3(256), 0, st, f, J, nx, ny, nz, ix, iy, iz);
     return hipGetLastError();
 }
 

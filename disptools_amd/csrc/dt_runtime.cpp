@@ -978,6 +978,7 @@ extern "C" int dt_jacobian_device(size_t nx, size_t ny, size_t nz, float dx, flo
 {
     if (nx < 3 || ny < 3 || nz < 3)
         DT_FAIL("every image dimension must be at least 3 voxels (the stencils read 3 samples)");
+    if (nx * ny * nz > 357913941u) DT_FAIL("image too large: at most 357913941 voxels per jacobian");
     if (!d_field || !d_J) DT_FAIL("null buffer");
     const float ix = (float)(1.0 / (double)dx), iy = (float)(1.0 / (double)dy), iz = (float)(1.0 / (double)dz);
     DT_HIP(launch_jacobian(d_field, d_J, (int)nx, (int)ny, (int)nz, ix, iy, iz, (hipStream_t)stream));

@@ -19,7 +19,7 @@ enum Kind : int { K_C4 = 0, K_C2 = 1, K_FWD = 2, K_BWD = 3 };
 // face and 2nd-order central elsewhere (:112-221); the penultimate layer uses
 // 2nd-order central on all axes (:223-241); everything else 4th-order central
 // (:87-110).
-__device__ __forceinline__ void classify(int x, int y, int z, int nx, int ny, int nz, int k[3])
+__host__ __device__ __forceinline__ void classify(int x, int y, int z, int nx, int ny, int nz, int k[3])
 {
     const bool ox = (x == 0) | (x == nx - 1);
     const bool oy = (y == 0) | (y == ny - 1);
@@ -47,7 +47,7 @@ __device__ __forceinline__ void stencil_weights(int kind, float w[5])
 
 // Derivative along one axis from the five samples f(-2) .. f(+2).
 template <typename CT>
-__device__ __forceinline__ CT dstencil(int kind, float m2, float m1, float c0, float p1, float p2, float inv)
+__host__ __device__ __forceinline__ CT dstencil(int kind, float m2, float m1, float c0, float p1, float p2, float inv)
 {
     switch (kind) {
     case K_C4:
@@ -62,14 +62,55 @@ __device__ __forceinline__ CT dstencil(int kind, float m2, float m1, float c0, f
 }
 
 template <typename CT>
-__device__ __forceinline__ CT dstencil_c4(float m2, float m1, float p1, float p2, float inv)
+__host__ __device__ __forceinline__ CT dstencil_c4(float m2, float m1, float p1, float p2, float inv)
 {
     return ((CT)(p1 - m1) * (CT)(2.0 / 3.0) - (CT)(p2 - m2) * (CT)(1.0 / 12.0)) * (CT)inv;
 }
 
+// dstencil<double> without branches, for lanes of one wave that hold
+// different stencil kinds.  Every kind is evaluated with the operations of
+// dstencil in the same order -- a - b*w as a + b*(-w), which rounds the same
+// -- from operands and weights picked by the kind:
+//   C4   ((p1 - m1) * 2/3 + (p2 - m2) * -1/12) * inv
+//   FWD  ((c0 * -1.5 + p1 * 2) + p2 * -0.5) * inv
+//   BWD  ((c0 * 1.5 + m1 * -2) + m2 * 0.5) * inv
+//   C2   (float)((p1 - m1) * inv) * 0.5
+// The selectors depend on the kind only; a voxel forms them once per axis
+// and shares them between the three field components.
+struct StencilSel {
+    bool c4, fwd, c2;
+    double wa, wb, wc;
+};
+
+__host__ __device__ __forceinline__ StencilSel stencil_sel(int kind)
+{
+    StencilSel s;
+    s.c4 = kind == K_C4;
+    s.fwd = kind == K_FWD;
+    s.c2 = kind == K_C2;
+    s.wa = s.c4 ? 2.0 / 3.0 : (s.fwd ? -1.5 : 1.5);
+    s.wb = s.c4 ? -(1.0 / 12.0) : (s.fwd ? 2.0 : -2.0);
+    s.wc = s.fwd ? -0.5 : 0.5;
+    return s;
+}
+
+__host__ __device__ __forceinline__ double dstencil_sel(const StencilSel &s, float m2, float m1, float c0, float p1,
+                                                        float p2, float inv)
+{
+    const float d1 = p1 - m1, d2 = p2 - m2;
+    const double a = (double)(s.c4 ? d1 : c0);
+    const double b = (double)(s.c4 ? d2 : (s.fwd ? p1 : m1));
+    const double c = (double)(s.fwd ? p2 : m2);
+    const double ab = a * s.wa + b * s.wb;
+    const double abc = ab + c * s.wc;
+    const double r = (s.c4 ? ab : abc) * (double)inv;
+    const double q = (double)(d1 * inv) * 0.5;
+    return s.c2 ? q : r;
+}
+
 // det(D + I), cofactor expansion along the first row (disptools.h:186-199).
 template <typename CT>
-__device__ __forceinline__ CT det_plus_identity(const CT d[3][3])
+__host__ __device__ __forceinline__ CT det_plus_identity(const CT d[3][3])
 {
     const CT a11 = d[0][0] + (CT)1.0, a12 = d[0][1], a13 = d[0][2];
     const CT a21 = d[1][0], a22 = d[1][1] + (CT)1.0, a23 = d[1][2];

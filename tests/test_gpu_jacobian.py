@@ -19,6 +19,8 @@ SHAPES = [
     ((9, 17, 33), (1.0, 2.0, 0.5)),
     ((40, 40, 40), (1.0, 1.0, 1.0)),
     ((70, 66, 130), (1.0, 1.0, 1.5)),
+    ((37, 70, 131), (1.0, 0.7, 1.3)),
+    ((128, 128, 128), (1.0, 1.0, 1.5)),
 ]
 
 
@@ -83,3 +85,10 @@ def test_regularise(gpu_lib):
     J = rng.random((20, 21, 22)).astype(np.float32)
     out = displacements.regularise(J, 0.3)
     assert np.array_equal(out, np.maximum(J, np.float32(0.3)))
+
+
+def test_jacobian_too_large_is_an_error(gpu_lib):
+    """More than 357913941 voxels (12 bytes per voxel beyond 32-bit byte
+    offsets): refused before any buffer is touched."""
+    with pytest.raises(gpu_lib.HipError, match="too large"):
+        gpu_lib.jacobian_device(1000, 1000, 1000, (1.0, 1.0, 1.0), 0, 0)
