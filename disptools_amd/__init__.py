@@ -21,12 +21,8 @@ __version__ = "0.4.1"
 
 def load_extension():
     """Import the in-tree ``_disptools`` extension (raises ImportError if unbuilt)."""
-    try:
-        # PyTorch-ROCm's bundled HIP runtime, when installed, must be loaded
-        # before the library binds one (a single runtime per process).
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    from . import _hip_runtime
+    _hip_runtime.preload()   # one HIP runtime per process, shared with torch
     try:
         return importlib.import_module("disptools_amd._disptools")
     except ImportError as e:

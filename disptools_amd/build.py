@@ -43,6 +43,7 @@ HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_pyramid.hip", "-ffp-contract=off"),
                ("dt_noalloc.hip", "-ffp-contract=off"),
                ("dt_runtime.cpp", "-ffp-contract=off"),
+               ("dt_levels.cpp", "-ffp-contract=off"),
                ("dt_engine.cpp", "-ffp-contract=off")]
 
 

@@ -160,6 +160,14 @@ void append_tiles_xcd(std::vector<Tile> &out, const std::vector<TileRun> &runs);
 // Checks the arguments of a solve (sets the error state and returns -1).
 int validate(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz, const dt_params *p);
 
+// One scale step of the image pyramid on the device (drawing.scale_image):
+// the [ncomp][nz][ny][nx] volume `in` (spacing dx, dy, dz) smoothed by the
+// recursive Gaussian of `sigma` physical units per axis, then resampled onto
+// mx x my x mz (linear, or nearest when `nearest`) into `out`; t0 and t1 are
+// scratch volumes of the input's size.  Enqueued on `st`.
+int scale_volume(int nx, int ny, int nz, int ncomp, float dx, float dy, float dz, const float *in, int mx, int my,
+                 int mz, float sigma, bool nearest, float *out, float *t0, float *t1, hipStream_t st);
+
 // Largest number of passes a solve may take before the host declares it
 // stuck (a line search that never terminates, e.g. alpha <= 1 or beta >= 1).
 long long max_passes(const dt_params &p);

@@ -33,8 +33,22 @@ hipError_t launch_grad_pass(int precision, int variant, const Inst *insts, const
                             int parity, hipStream_t st);
 // resident blocks per CU of a variant (from the compiled resource usage)
 int grad_pass_blocks_per_cu(int precision, int variant);
-hipError_t launch_smooth_axis(const float *in, float *out, int nx, int ny, int nz, int ncomp, int axis,
-                              const float *d_w, int radius, hipStream_t st);
+// Coefficients of the recursive Gaussian along one axis (dt_pyramid.hip):
+// causal numerator n[0..3], anti-causal numerator m[0..3] (z^1 .. z^4),
+// denominator 1 + d[0..3] z^-1 .. z^-4, the steady states per unit input
+// (nsum = sum n / (1 + sum d), msum likewise) and the DC normalisation.
+struct GaussCoef {
+    double n[4], m[4], d[4];
+    double nsum, msum, scale;
+};
+// Deriche's fourth-order recursive Gaussian of standard deviation `sigma`
+// samples (sigma > 0).
+GaussCoef gauss_coefficients(double sigma);
+hipError_t launch_gauss_axis(const float *in, float *out, int nx, int ny, int nz, int ncomp, int axis,
+                             const GaussCoef &g, hipStream_t st);
+hipError_t launch_nonzero_u8(const float *in, unsigned char *out, long long n, hipStream_t st);
+// to_xyz3: [3][n] -> [n][3]; else [n][3] -> [3][n]
+hipError_t launch_interleave(const float *in, float *out, long long n, bool to_xyz3, hipStream_t st);
 hipError_t launch_resample(const float *in, float *out, int nx, int ny, int nz, int mx, int my, int mz, int ncomp,
                            float rx, float ry, float rz, int nearest, hipStream_t st);
 // Greedy pass variants (dt_greedy_stream.hip): GREEDY_FUSED = 32 x 16

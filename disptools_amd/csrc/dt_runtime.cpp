@@ -1054,41 +1054,14 @@ extern "C" int dt_scale_volume_device(size_t nx, size_t ny, size_t nz, size_t nc
     const size_t n = nx * ny * nz * ncomp;
     if (n >= (1ull << 40)) DT_FAIL("volume too large");
     hipStream_t st = (hipStream_t)stream;
-    // Gaussian taps per axis, normalised
-    const float sp[3] = {dx, dy, dz};
-    constexpr int MAXR = 128;
-    std::vector<float> taps(3 * (2 * MAXR + 1), 0.0f);
-    int radius[3];
-    for (int a = 0; a < 3; ++a) {
-        const double sv = sigma / sp[a];
-        radius[a] = sv > 0.0 ? std::min(MAXR, (int)std::ceil(4.0 * sv)) : 0;
-        double sum = 0.0;
-        for (int k = -radius[a]; k <= radius[a]; ++k) {
-            const double w = sv > 0.0 ? std::exp(-0.5 * (k / sv) * (k / sv)) : 1.0;
-            taps[a * (2 * MAXR + 1) + k + radius[a]] = (float)w;
-            sum += w;
-        }
-        for (int k = 0; k <= 2 * radius[a]; ++k) taps[a * (2 * MAXR + 1) + k] = (float)(taps[a * (2 * MAXR + 1) + k] / sum);
-    }
     Arena *A = nullptr;
-    const size_t tbytes = align_up(taps.size() * sizeof(float), 256);
-    if (arena_get(tbytes + 2 * align_up(n * sizeof(float), 256) + 512, &A)) return -1;
+    if (arena_get(2 * align_up(n * sizeof(float), 256) + 512, &A)) return -1;
     Carver cv(A->ptr);
-    float *d_taps = cv.take<float>(taps.size());
     float *t0 = cv.take<float>(n);
     float *t1 = cv.take<float>(n);
-    DT_HIP(hipMemcpyAsync(d_taps, taps.data(), taps.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    const float *src = d_in;
-    float *bufs[2] = {t0, t1};
-    for (int a = 0; a < 3; ++a) {
-        float *dst = bufs[a & 1];
-        DT_HIP(launch_smooth_axis(src, dst, (int)nx, (int)ny, (int)nz, (int)ncomp, a, d_taps + a * (2 * MAXR + 1),
-                                  radius[a], st));
-        src = dst;
-    }
-    DT_HIP(launch_resample(src, d_out, (int)nx, (int)ny, (int)nz, (int)mx, (int)my, (int)mz, (int)ncomp,
-                           (float)((double)nx / (double)mx), (float)((double)ny / (double)my),
-                           (float)((double)nz / (double)mz), nearest ? 1 : 0, st));
+    if (scale_volume((int)nx, (int)ny, (int)nz, (int)ncomp, dx, dy, dz, d_in, (int)mx, (int)my, (int)mz, sigma,
+                     nearest != 0, d_out, t0, t1, st))
+        return -1;
     // the arena is reused by the next call: finish before returning
     DT_HIP(hipStreamSynchronize(st));
     return 0;

@@ -14,15 +14,15 @@ Behaviour of this build that differs from the reference:
   HIP device through the reference's CPU-named entry points
   (``generate_displacement_gradient`` / ``_greedy``); there is no CPU solver;
 * the image pyramid (``levels > 1``, and an ``initial_guess``, which the
-  reference also passes through ``drawing.scale_image``) is built on the
-  device by ``dt_scale_volume_device``: a Gaussian of sigma 2 (physical
-  units) with taps out to 4 sigma and replicated edges, then the same
-  resampling as SimpleITK's ResampleImageFilter (same origin and extent,
-  linear or nearest-neighbour, 0 outside).  The reference smooths with
-  SimpleITK's recursive (IIR) Gaussian, which is not installed here; the
-  levels therefore agree with the reference's only to that filter's
-  approximation of a Gaussian (parity unpinned).  All levels, masks and
-  intermediate fields stay in device memory.
+  reference also passes through ``drawing.scale_image``) is built and solved
+  on the device by ``dt_displacement_levels``: the recursive Gaussian of
+  sigma 2 (physical units) that SimpleITK's SmoothingRecursiveGaussian
+  documents (Deriche's fourth-order filter, restated from the published
+  method: SimpleITK is not installed here, so agreement with it is pinned by
+  the filter's mathematics, not bit for bit), then the resampling of
+  ResampleImageFilter (same origin and extent, linear or nearest-neighbour,
+  0 outside).  All levels, masks and intermediate fields stay in device
+  memory; numpy arrays in and out, no torch.
 """
 
 import logging
@@ -151,74 +151,29 @@ def pyramid_sizes(size_xyz, levels):
     return sizes
 
 
-def _scale(d_in, spacing, size_xyz, nearest=False, sigma=2.0):
-    """drawing.scale_image on a device volume [ncomp, nz, ny, nx]: returns
-    the resampled volume (same origin and physical extent) and its spacing."""
-    import torch
-    from . import native
-    ncomp, nz, ny, nx = d_in.shape
-    mx, my, mz = size_xyz
-    if min(mx, my, mz) < 1:
-        raise ValueError("pyramid level of size %r is empty" % (size_xyz,))
-    out = torch.empty((ncomp, mz, my, mx), dtype=torch.float32, device=d_in.device)
-    native.scale_volume_device(nx, ny, nz, ncomp, spacing, d_in.data_ptr(), (mx, my, mz), sigma, nearest,
-                               out.data_ptr(), torch.cuda.current_stream(d_in.device).cuda_stream)
-    new_spacing = tuple(s * n / m for s, n, m in zip(spacing, (nx, ny, nz), (mx, my, mz)))
-    return out, new_spacing
-
-
 def _pyramid_displacement(J, M, guess, spacing, levels, it_max, parameters):
-    """Multi-resolution solve on the device (reference displacements.py:449-483):
-    every coarser level is scaled from the finest padded target and mask;
-    each level starts from the previous level's field scaled onto its grid
-    (the initial guess, if any, for the coarsest)."""
-    import torch
+    """Multi-resolution solve (reference displacements.py:449-483) on the
+    device through dt_displacement_levels: every coarser level is scaled from
+    the finest padded target and mask; each level starts from the previous
+    level's field scaled onto its grid (the initial guess, if any, for the
+    coarsest).  Host arrays in and out; the library allocates and frees the
+    device buffers itself."""
     from . import native
     gpu_id = int(parameters["gpu_id"])
     if gpu_id >= 0 and not native.set_device(gpu_id):
         raise ValueError("HIP device %d is not available" % gpu_id)
     # gpu_id < 0: the current device (the reference's CPU solver, which this
     # build does not have, runs there instead)
-    dev = torch.device("cuda", gpu_id if gpu_id >= 0 else torch.cuda.current_device())
-    nz, ny, nx = J.shape
-    sizes = pyramid_sizes((nx, ny, nz), levels)
-    dJ = torch.from_numpy(np.ascontiguousarray(J, np.float32)[None]).to(dev)
-    dM = torch.from_numpy(np.ascontiguousarray(M != 0, np.float32)[None]).to(dev)
-    J_levels, M_levels, sp_levels = [dJ[0]], [dM[0]], [tuple(spacing)]
-    for size in sizes[1:]:
-        j, sp_l = _scale(dJ, spacing, size)
-        m, _ = _scale(dM, spacing, size, nearest=True)
-        J_levels.append(j[0])
-        M_levels.append(m[0])
-        sp_levels.append(sp_l)
-    field, f_sp = None, tuple(spacing)
-    if guess is not None:
-        # [z, y, x, 3] -> [3, z, y, x] on the device
-        g = torch.from_numpy(np.ascontiguousarray(guess, np.float32)).to(dev)
-        field = g.permute(3, 0, 1, 2).contiguous()
+    for size in pyramid_sizes(tuple(reversed(J.shape)), levels):
+        logging.info('Size %s' % str(size))
+    algorithm = parameters["algorithm"]
     params = {k: v for k, v in parameters.items() if k not in ("algorithm", "gpu_id")}
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    # it_max[k] belongs to the k-th level from the coarsest (reference :476 zips
-    # the reversed pyramid with it_max)
-    for k, level in enumerate(reversed(range(levels))):
-        mx, my, mz = sizes[level]
-        logging.info('Size %s' % str(sizes[level]))
-        if field is not None:
-            field, _ = _scale(field, f_sp, sizes[level])
-        else:
-            field = torch.zeros((3, mz, my, mx), dtype=torch.float32, device=dev)
-        T = J_levels[level].contiguous()
-        Mb = (M_levels[level] != 0).to(torch.uint8).contiguous()
-        field = field.contiguous()
-        if parameters["algorithm"] == "matching":
-            native.matching_device(mx, my, mz, sp_levels[level], T.data_ptr(), field.data_ptr(), stream=stream,
-                                   it_max=it_max[k], **params)
-        else:
-            native.solve_device(parameters["algorithm"], mx, my, mz, sp_levels[level], T.data_ptr(),
-                                Mb.data_ptr(), field.data_ptr(), stream=stream, it_max=it_max[k], **params)
-        f_sp = sp_levels[level]
-    # [3, z, y, x] -> [z, y, x, 3] on the device, then one download
-    return field.permute(1, 2, 3, 0).contiguous().cpu().numpy()
+    if algorithm == "matching":
+        params = {k: params[k] for k in ("epsilon", "delta", "zeta")}
+    # the reference casts the mask to float before scaling it (displacements.py:407)
+    field, _ = native.displacement_levels(algorithm, J, np.asarray(M, np.float32), guess, spacing, levels,
+                                          list(it_max), **params)
+    return field
 
 
 def displacement(

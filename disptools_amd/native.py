@@ -11,6 +11,8 @@ import re
 
 import numpy as np
 
+from . import _hip_runtime
+
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libdisptools_hip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "disptools_hip.h")
@@ -18,14 +20,8 @@ HEADER = os.path.join(os.path.dirname(PKG), "include", "disptools_hip.h")
 if not os.path.exists(LIB_PATH):
     raise ImportError("%s is not built; run `python -m disptools_amd.build`" % LIB_PATH)
 
-# PyTorch-ROCm ships its own HIP runtime (same soname as /opt/rocm's).  Loaded
-# first, it is the one runtime of the process and this library binds to it,
-# so device pointers of torch tensors are valid here; loaded after this
-# library, torch would find the other runtime already in place and no device.
-try:
-    import torch  # noqa: F401
-except ImportError:
-    pass
+# one HIP runtime per process, shared with torch when it is installed
+_hip_runtime.preload()
 
 lib = ctypes.CDLL(LIB_PATH)
 
@@ -195,6 +191,9 @@ class TuneResult(ctypes.Structure):
 lib.dt_scale_volume_device.argtypes = [_sz, _sz, _sz, _sz, _f, _f, _f, _vp, _sz, _sz, _sz, _f, ctypes.c_int, _vp,
                                        _vp]
 lib.dt_scale_volume_device.restype = ctypes.c_int
+lib.dt_displacement_levels.argtypes = [ctypes.c_int, _sz, _sz, _sz, _f, _f, _f, _vp, _vp, _vp, _sz, _sz, _sz,
+                                       ctypes.c_int, _vp, _vp, _vp, _vp]
+lib.dt_displacement_levels.restype = ctypes.c_int
 
 lib.dt_tune_gradient.argtypes = [_sz, _sz, _sz, _f, _f, _f, ctypes.c_int, ctypes.POINTER(TuneResult)]
 lib.dt_tune_gradient.restype = ctypes.c_int
@@ -350,6 +349,38 @@ def matching_device(nx, ny, nz, spacing, d_target, d_field, stream=0, it_max=500
     if rc != 0:
         raise HipError(last_error())
     return info.as_dict(), trace[:min(info.trace_len, trace_cap)].copy()
+
+
+def displacement_levels(algorithm, J, mask, guess, spacing, levels, it_max, **params):
+    """Multi-resolution solve on host arrays (dt_displacement_levels): J and
+    mask [nz, ny, nx] (mask as float), guess [gz, gy, gx, 3] or None,
+    it_max a list of `levels` limits from the coarsest level.  Returns the
+    field [nz, ny, nx, 3] (float32) and the finest level's record."""
+    alg = {"gradient": GRADIENT, "greedy": GREEDY, "matching": MATCHING}[algorithm]
+    J = np.ascontiguousarray(J, dtype=np.float32)
+    mask = np.ascontiguousarray(mask, dtype=np.float32)
+    if mask.shape != J.shape:
+        raise ValueError("the shapes of the target and the mask must agree")
+    nz, ny, nx = J.shape
+    gz = gy = gx = 0
+    if guess is not None:
+        guess = np.ascontiguousarray(guess, dtype=np.float32)
+        if guess.ndim != 4 or guess.shape[3] != 3:
+            raise ValueError("the initial guess must be [z, y, x, 3]")
+        gz, gy, gx = guess.shape[:3]
+    if len(it_max) != levels:
+        raise ValueError("len(it_max) should equal the value of `levels`")
+    limits = (ctypes.c_longlong * levels)(*[int(v) for v in it_max])
+    p = make_params(**params)
+    info = SolveInfo()
+    field = np.empty((nz, ny, nx, 3), np.float32)
+    rc = lib.dt_displacement_levels(alg, nx, ny, nz, *map(float, spacing), _host_ptr(J), _host_ptr(mask),
+                                    _host_ptr(guess) if guess is not None else None, gx, gy, gz, int(levels),
+                                    ctypes.cast(limits, ctypes.c_void_p), ctypes.byref(p), _host_ptr(field),
+                                    ctypes.byref(info))
+    if rc != 0:
+        raise HipError(last_error())
+    return field, info.as_dict()
 
 
 def scale_volume_device(nx, ny, nz, ncomp, spacing, d_in, size_out, sigma, nearest, d_out, stream=0):

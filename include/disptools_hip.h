@@ -237,15 +237,38 @@ int dt_solve_host(int algorithm, size_t nx, size_t ny, size_t nz,
 
 /* Scale step of the image pyramid (the reference's drawing.scale_image,
  * disptools/drawing.py:23-69, there with SimpleITK): smooths a
- * [ncomp][nz][ny][nx] float volume with spacing (dx, dy, dz) by a Gaussian of
- * standard deviation `sigma` in physical units along each axis (taps out to
- * 4 sigma, edge samples replicated), then resamples it onto an mx x my x mz
- * grid with the same origin and physical extent (spacing d * n / m): linear
- * interpolation, or nearest neighbour when `nearest` (masks), 0 outside the
- * input.  Device pointers; runs on `stream`.  Returns 0 or -1. */
+ * [ncomp][nz][ny][nx] float volume with spacing (dx, dy, dz) by the recursive
+ * Gaussian of standard deviation `sigma` in physical units along each axis
+ * (Deriche's fourth-order filter, which SimpleITK's SmoothingRecursiveGaussian
+ * implements; unit DC gain, the signal continued by its end samples), then
+ * resamples it onto an mx x my x mz grid with the same origin and physical
+ * extent (spacing d * n / m): linear interpolation, or nearest neighbour when
+ * `nearest` (masks), 0 outside the input.  Device pointers; runs on
+ * `stream`.  Returns 0 or -1. */
 int dt_scale_volume_device(size_t nx, size_t ny, size_t nz, size_t ncomp, float dx, float dy, float dz,
                            const float *d_in, size_t mx, size_t my, size_t mz, float sigma, int nearest,
                            float *d_out, void *stream);
+
+/* The multi-resolution solve of displacement() (disptools/displacements.py:
+ * 454-483), host buffers in and out, everything between on the device:
+ * `levels` grids, each half the previous one (integer division); every
+ * coarser target and mask is scale_image of the finest (linear, nearest);
+ * the coarsest level starts from the initial guess scaled onto its grid, or
+ * zero, every finer one from the previous level's field scaled onto its own;
+ * it_max[k] is the iteration limit of the k-th level from the coarsest.
+ *   J      [nz][ny][nx] target (padded as displacement() pads it)
+ *   mask   [nz][ny][nx] float (the reference casts the mask to float; a
+ *          voxel takes part where the scaled mask is non-zero)
+ *   guess  [gz][gy][gx][3] initial guess with spacing (dx, dy, dz), or NULL
+ *   field  [nz][ny][nx][3] result
+ * algorithm: DT_GRADIENT, DT_GREEDY or DT_MATCHING (epsilon, delta, zeta of
+ * `params`).  `info` (may be NULL) receives the finest level's record.
+ * Returns 0 or -1.  No counterpart in the reference's C library (its Python
+ * driver composes SimpleITK and the C solver). */
+int dt_displacement_levels(int algorithm, size_t nx, size_t ny, size_t nz, float dx, float dy, float dz,
+                           const float *J, const float *mask, const float *guess, size_t gx, size_t gy, size_t gz,
+                           int levels, const long long *it_max, const dt_params *params, float *field,
+                           dt_solve_info *info);
 
 /* ------------------------------------------------------------------------ */
 /* Autotuner of the gradient pass (no counterpart in the reference).        */

@@ -170,3 +170,20 @@ def test_algorithm_values_match_the_header():
     assert values == {"GRADIENT": 0, "GREEDY": 1, "MATCHING": 2}
     assert native.ALGORITHMS == {"gradient": values["GRADIENT"], "greedy": values["GREEDY"]}
     assert native.ENGINE_ALGORITHMS == {k.lower(): v for k, v in values.items()}
+
+
+def test_importing_the_api_does_not_import_torch():
+    """The Python API (extension, ctypes bindings, displacement()) loads
+    without torch; torch's HIP runtime, when installed, is mapped by path so
+    that a torch imported later shares it."""
+    import subprocess
+    import sys
+    code = ("import sys\n"
+            "import disptools_amd.displacements, disptools_amd.native\n"
+            "assert 'torch' not in sys.modules\n"
+            "maps = open('/proc/self/maps').read()\n"
+            "hip = sorted({l.split()[-1] for l in maps.splitlines() if 'libamdhip64' in l})\n"
+            "assert len(hip) == 1, hip\n"
+            "print(hip[0])\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
