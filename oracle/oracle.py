@@ -86,7 +86,8 @@ def _ptr(a):
 
 
 def oracle_lib_path(bits=32):
-    return os.path.join(HERE, "build", "liboracle_f%d.so" % bits)
+    # DT_ORACLE_BUILD: another build of the same sources (the sanitizer test's)
+    return os.path.join(os.environ.get("DT_ORACLE_BUILD", os.path.join(HERE, "build")), "liboracle_f%d.so" % bits)
 
 
 def reference_lib_path(bits=32):
