@@ -237,10 +237,11 @@ int engine_tz(const dt_engine *e, int nz)
 // estimated pass time.  A block of depth d costs d + O plane steps (O: the
 // halo planes of the pipeline and the block's fixed work -- control step,
 // sums -- in plane-step units); the CUs run the blocks in rounds, so a pass
-// of blocks of total cost W and largest cost c takes about
-// ceil(W / (CUs * c)) * c.  Small D fills the CUs when few instances remain,
-// large D saves halo planes when many do.  The results do not depend on the
-// depths (exact sums), so an instance may be re-tiled between any two passes.
+// of up to two rounds of blocks of largest cost c takes ceil(blocks / CUs) * c,
+// a longer one about ceil(W / (CUs * c)) * c for blocks of total cost W.
+// Small D fills the CUs when few instances remain, large D saves halo planes
+// when many do.  The results do not depend on the depths (exact sums), so an
+// instance may be re-tiled between any two passes.
 void retile_lane(dt_engine *e, Lane &L)
 {
     const int cus = device_cus();
@@ -249,6 +250,7 @@ void retile_lane(dt_engine *e, Lane &L)
     double best_cost = 1e300;
     for (int D = 4; D <= 64; D += 4) {
         double work = 0.0;
+        long long blocks = 0;
         int cmax = 0;
         for (int s = 0; s < (int)e->slots.size(); ++s) {
             const Slot &sl = e->slots[s];
@@ -257,10 +259,14 @@ void retile_lane(dt_engine *e, Lane &L)
             const long long tiles = (long long)sl.P.ntx * sl.P.nty * ((sl.P.nz + d - 1) / d);
             const int c = std::min(d, sl.P.nz) + O;
             work += (double)tiles * c;
+            blocks += tiles;
             cmax = std::max(cmax, c);
         }
         if (cmax == 0) return;
-        const double cost = std::ceil(work / ((double)cus * cmax)) * cmax;
+        // a few rounds: whole rounds of the deepest block (257 blocks take
+        // two rounds, not 1.004); many: the work spread over the CUs
+        const double cost = blocks <= 2ll * cus ? (double)((blocks + cus - 1) / cus) * cmax
+                                                : std::ceil(work / ((double)cus * cmax)) * cmax;
         if (cost < best_cost * 0.999) { best_cost = cost; best_d = D; }
     }
     for (int s = 0; s < (int)e->slots.size(); ++s) {
