@@ -96,7 +96,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="it_max of the CPU baseline solves (0 to skip the baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy", "pyramid", "jacobian"), default="solve")
+    ap.add_argument("--mode", choices=("solve", "batch", "serve", "copy", "pyramid", "jacobian", "sphere"), default="solve")
     ap.add_argument("--algorithm", choices=("gradient", "greedy", "matching"), default="gradient",
                     help="solver of the solve mode (the headline workload is gradient)")
     ap.add_argument("--batch", type=int, default=64,
@@ -479,6 +479,81 @@ def jacobian_bench(sides=(128, 256), reps=50):
     return out
 
 
+def sphere_bench(threads, reps=20, min_cpu_seconds=5.0):
+    """The reference's CPU-runnable configuration: a 40^3 sphere target
+    (radius 10, 1.1 inside, 1.0 outside; mask = the sphere), spacing 1, zero
+    init, gradient, it_max 500.  GPU: dt_solve_device on resident buffers,
+    `reps` solves timed.  CPU: the reference C build (oracle/_ref; the
+    restatement when it is missing) on `threads` threads and on one, same
+    inputs, repeated for about `min_cpu_seconds` each.  Each side reports
+    iterations, stop reason, time per solve, voxel-iters/s and the final
+    masked max |J - T| of its field (FP32 jacobian of the oracle)."""
+    import numpy as np
+    import torch
+    from disptools_amd import native
+    from disptools_amd.workloads import sphere_case
+    from oracle.oracle import Oracle, Reference, reference_available, REASON_NAMES
+
+    T, M, sp = sphere_case(40)
+    n = T.size
+    o32 = Oracle(32)
+
+    def masked_max(field):
+        _, mx, _ = o32.compute_error(T, o32.jacobian(np.ascontiguousarray(field, np.float32), sp), M, 0.2)
+        return mx
+
+    dev = torch.device("cuda:0")
+    dT = torch.from_numpy(T).to(dev)
+    dM = torch.from_numpy(M.astype(np.uint8)).to(dev)
+    dF = torch.zeros((3,) + T.shape, dtype=torch.float32, device=dev)
+    sh = torch.cuda.current_stream(dev).cuda_stream
+
+    def gpu_solve():
+        dF.zero_()
+        info, _ = native.solve_device("gradient", 40, 40, 40, sp, dT.data_ptr(), dM.data_ptr(), dF.data_ptr(),
+                                      stream=sh, it_max=500)
+        return info
+
+    gpu_solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        info = gpu_solve()
+    torch.cuda.synchronize()
+    g_s = (time.perf_counter() - t0) / reps
+    gpu = {"iterations": info["iterations"], "reason": native.REASONS[info["reason"]], "seconds_per_solve": g_s,
+           "voxel_iters_per_s": n * info["iterations"] / g_s, "final_masked_max_error": masked_max(dF.cpu().numpy())}
+
+    kind = "reference" if reference_available(32) else "port"
+    ref = Reference(32) if kind == "reference" else None
+
+    def cpu(nthreads):
+        o32.set_threads(nthreads)   # shared OpenMP runtime: applies to the reference build too
+        done = reps_c = 0
+        dt = 0.0
+        while reps_c == 0 or (dt < min_cpu_seconds and reps_c < 50):
+            t0 = time.perf_counter()
+            if ref is not None:
+                f, inf = ref.solve("gradient", T, M, sp, it_max=500, capture=True)
+                its, reason = inf["iterations"], inf["reason"]
+            else:
+                f, inf = o32.solve("gradient", T, M, sp, it_max=500)
+                its, reason = inf["iterations"], inf["reason"]
+            dt += time.perf_counter() - t0
+            done += its
+            reps_c += 1
+        o32.set_threads(1)
+        return {"threads": nthreads, "iterations": its, "reason": REASON_NAMES.get(reason, reason),
+                "seconds_per_solve": dt / reps_c, "voxel_iters_per_s": n * done / dt,
+                "final_masked_max_error": masked_max(f), "solves": reps_c}
+
+    out = {"workload": "40^3 sphere (radius 10, 1.1 inside / 1.0 outside, mask = sphere), spacing 1, zero init, "
+                       "gradient, it_max 500", "gpu": gpu, "cpu_kind": kind, "cpu": cpu(threads)}
+    if threads > 1:
+        out["cpu_one_thread"] = cpu(1)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -490,6 +565,15 @@ def main():
         return
     if args.mode == "pyramid":
         print(json.dumps({"mode": "pyramid", **pyramid_bench(args.it_max)}))
+        return
+    if args.mode == "sphere":
+        from disptools_amd import native
+        if not native.set_device(0):
+            raise RuntimeError("no HIP device")
+        cores, cores_source = physical_cores()
+        out = sphere_bench(args.cpu_threads or cores)
+        out["cores_source"] = cores_source if not args.cpu_threads else "--cpu-threads"
+        print(json.dumps({"mode": "sphere", **out}))
         return
     if args.mode == "jacobian":
         from disptools_amd import native
