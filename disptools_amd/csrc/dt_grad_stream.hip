@@ -21,6 +21,9 @@
 //    parity: one barrier per plane separates every write from its reads;
 //  * the rest of the halo (rings 2 and 3, ring-1 corners) is spread over all
 //    threads;
+//  * trips whose two planes lie inside the z-chunk and the volume and off
+//    the z faces -- all but the first and last few of a chunk -- run a step
+//    with those tests folded away (fewer scalar instructions and branches);
 //  * no register that a load writes is carried across the loop's back edge:
 //    a plane's loads are resolved and committed to the LDS ring at the top of
 //    the next step (the ring's third plane is what lets the commit happen at
@@ -28,6 +31,8 @@
 //    body is two steps).  A loaded register copied at the back edge makes the
 //    step wait for the load it has just issued, which turns the one-plane
 //    prefetch into a full memory round trip per step.
+#include <type_traits>
+
 #include "dt_device.hpp"
 #include "dt_kernels.hpp"
 #include "dt_pass_common.hpp"
@@ -87,12 +92,13 @@ __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 template <typename CT>
 __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, int zz, int gx, int gy,
                                                const float (&sm)[3][3][4], const float (&v0)[3], float T,
-                                               bool in_mask, bool count, double &Egrp, float &mx, CT (&W)[9])
+                                               bool in_mask, bool count, double &Egrp, float &mx, CT (&W)[9],
+                                               bool zint)
 {
     // planes away from the z faces: the column's own coefficients (no
     // wave-wide branch on boundary lanes); the four face planes: region map
-    const float dJ = (zz >= 2 && zz <= P.nz - 3) ? jacobian_delta_coef<CT>(K, sm, v0)
-                                                : jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
+    const float dJ = zint ? jacobian_delta_coef<CT>(K, sm, v0)
+                          : jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     const float ec = count ? e : 0.0f;
@@ -224,8 +230,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     };
     // trial values of a plane (zero outside the volume)
     struct Val { float v[SLOTS]; };
-    auto resolve = [&](int gz, const Raw &r, Val &o) {
-        const bool zin = gz >= 0 && gz < nz;
+    auto resolve = [&](int gz, const Raw &r, Val &o, bool known_in) {
+        const bool zin = known_in || (gz >= 0 && gz < nz);
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             float v = r.a[k];
@@ -265,17 +271,17 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         fetch(zs - 1, r1);
         fetch_tm(zs, tma);
         Val v;
-        resolve(zs - 2, r0, v);
+        resolve(zs - 2, r0, v, false);
         col[1][0] = v.v[0]; col[1][1] = v.v[1]; col[1][2] = v.v[2];
         fetch(zs, r0);
-        resolve(zs - 1, r1, v);
+        resolve(zs - 1, r1, v, false);
         col[2][0] = v.v[0]; col[2][1] = v.v[1]; col[2][2] = v.v[2];
         fetch(zs + 1, r1);
-        resolve(zs, r0, v);
+        resolve(zs, r0, v, false);
         col[3][0] = v.v[0]; col[3][1] = v.v[1]; col[3][2] = v.v[2];
         commit(0, v);
         fetch(zs + 2, rn);
-        resolve(zs + 1, r1, v);
+        resolve(zs + 1, r1, v, false);
         col[4][0] = v.v[0]; col[4][1] = v.v[1]; col[4][2] = v.v[2];
         commit(1, v);
         col[0][0] = col[0][1] = col[0][2] = 0.0f;
@@ -313,7 +319,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 
     // step zz: plane zz+2 joins the window and the LDS ring, stage A on plane
     // zz (ring slot sa), stage B on plane zz-1
-    auto step = [&](int zz, int sa, TM &cur, TM &nxt) {
+    auto step = [&](auto fast, int zz, int sa, TM &cur, TM &nxt) {
+        constexpr bool FAST = decltype(fast)::value;
         // The loads of plane zz+2 and of this plane's target / mask were issued
         // one step ago.  An empty asm that takes them here keeps the compiler
         // from hoisting their first use above the barrier that ended the last
@@ -326,7 +333,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         asm volatile("" : "+v"(cur.t));
         {
             Val vz;
-            resolve(zz + 2, rn, vz);
+            resolve(zz + 2, rn, vz, FAST);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 col[0][c] = col[1][c]; col[1][c] = col[2][c]; col[2][c] = col[3][c]; col[3][c] = col[4][c];
@@ -341,7 +348,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         // (A) plane zz at the primary point (computed at points outside the
         // volume too: nothing reads their W, their error is not counted)
         CT wz_p[3] = {0, 0, 0};
-        if (zz >= 0 && zz < nz) {
+        if (FAST || (zz >= 0 && zz < nz)) {
             const float *R0 = &S.xs[sa][0][0] + pi;
             float sm[3][3][4];
 #pragma unroll
@@ -351,9 +358,10 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                 sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
                 sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
             }
-            const bool count = own && pin && zz >= z0 && zz < z1;
+            const bool count = own && pin && (FAST || (zz >= z0 && zz < z1));
             CT W[9];
-            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
+            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W,
+                               FAST || (zz >= 2 && zz <= nz - 3));
             const int pz = zz & 1;
             if (own) {
 #pragma unroll
@@ -370,15 +378,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 #pragma unroll
             for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
         }
-        if (zz >= z0 && zz < z1 && ((zz & 3) == 3 || zz == nz - 1)) flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
+        if (FAST ? (zz & 3) == 3 : (zz >= z0 && zz < z1 && ((zz & 3) == 3 || zz == nz - 1)))
+            flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
 
         // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
-        if (own && zb >= z0 && zb < z1) {
+        if (own && (FAST || (zb >= z0 && zb < z1))) {
             const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-            if (zb >= 1 && zb <= nz - 2) {
+            if (FAST || (zb >= 1 && zb <= nz - 2)) {
                 const CT dxx = (CT)S.wx[pb][0][ly][lx] - (CT)S.wx[pb][0][ly][lx + 2];
                 const CT dyx = (CT)S.wx[pb][1][ly][lx] - (CT)S.wx[pb][1][ly][lx + 2];
                 const CT dzx = (CT)S.wx[pb][2][ly][lx] - (CT)S.wx[pb][2][ly][lx + 2];
@@ -397,7 +406,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             pend.on = pin;
             pend.n[0] = ux - eta_next * f0; pend.n[1] = uy - eta_next * f1; pend.n[2] = uz - eta_next * f2;
         }
-        if (zb >= z0 && zb < z1 && ((zb & 3) == 3 || zb == nz - 1)) flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
+        if (FAST ? (zb & 3) == 3 : (zb >= z0 && zb < z1 && ((zb & 3) == 3 || zb == nz - 1)))
+            flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { wz_m[a] = wz_c[a]; wz_c[a] = wz_p[a]; }
         __syncthreads();
@@ -407,12 +417,25 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // the trip is not unrolled further (more steps in flight would raise the
     // live registers)
     int sa = 0;
+    // trips (zz, zz + 1) whose planes are inside the chunk and the volume and
+    // off the z faces take the step with those tests folded away
+    const int flo = max(z0 + 1, 2), fhi = min(z1 - 1, nz - 3);
+    const int t_lo = (flo - zs + 1) >> 1;
+    const int t_hi = fhi - 1 - zs >= 0 ? (fhi - 1 - zs) >> 1 : -1;
+    int t = 0;
 #pragma unroll 1
-    for (int zz = zs; zz < z1; zz += 2) {
-        step(zz, sa, tma, tmb);
-        sa = sa == 2 ? 0 : sa + 1;
-        step(zz + 1, sa, tmb, tma);
-        sa = sa == 2 ? 0 : sa + 1;
+    for (int zz = zs; zz < z1; zz += 2, ++t) {
+        if (t >= t_lo && t <= t_hi) {
+            step(std::true_type{}, zz, sa, tma, tmb);
+            sa = sa == 2 ? 0 : sa + 1;
+            step(std::true_type{}, zz + 1, sa, tmb, tma);
+            sa = sa == 2 ? 0 : sa + 1;
+        } else {
+            step(std::false_type{}, zz, sa, tma, tmb);
+            sa = sa == 2 ? 0 : sa + 1;
+            step(std::false_type{}, zz + 1, sa, tmb, tma);
+            sa = sa == 2 ? 0 : sa + 1;
+        }
     }
     // the last step's outputs: shift the window as a next step would
 #pragma unroll
