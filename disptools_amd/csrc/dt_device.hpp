@@ -454,6 +454,10 @@ __device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c, unsi
             c->acc_zero = prev;
         }
         c->evaluated = c->active;
+        // the pass in which the solve ends copies its result into buf[0]
+        // (once: the copy of the next pass records it as done)
+        c->copy_now = (!c->active && c->phase == PH_DONE && !c->result_copied) ? 1 : 0;
+        if (c->copy_now) c->result_copied = 1;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -491,5 +495,33 @@ __device__ __forceinline__ TileCoord decode_tile(const Inst &I, const Tile &t)
 
 // The instance's first tile: the one that stores the next control block.
 __device__ __forceinline__ bool leader_tile(const Tile &t) { return t.tile == 0 && t.z0 == 0; }
+
+// Run by every block of the pass in which a solve ends (copy_now): the
+// block's tile of the result buffer the controller names is copied into the
+// caller's field, buf[0] (the reference returns field_[old],
+// src/displacement_field_gradient.c:421).  The tiles of a pass partition the
+// volume, so the whole result moves without a launch of its own.  Nothing to
+// do when the result is already in buf[0].
+template <int TX, int TY, int NT>
+__device__ void copy_result_tile(const Inst &I, const Ctrl *C, const TileCoord &tc)
+{
+    const int rb = __builtin_amdgcn_readfirstlane(C->result_buf);
+    if (rb <= 0 || rb >= NBUF) return;
+    const float *__restrict__ src = I.buf[rb];
+    float *__restrict__ dst = I.buf[0];
+    const long long plane = (long long)I.nx * I.ny;
+    for (int t = threadIdx.x; t < TX * TY; t += NT) {
+        const int x = tc.x0 + t % TX, y = tc.y0 + t / TX;
+        if (x >= I.nx || y >= I.ny) continue;
+        const long long col = (long long)y * I.nx + x;
+        for (int z = tc.z0; z < tc.z1; ++z) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const long long i = c * I.n + (long long)z * plane + col;
+                dst[i] = src[i];
+            }
+        }
+    }
+}
 
 }  // namespace dt

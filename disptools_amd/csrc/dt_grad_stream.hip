@@ -460,7 +460,10 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, in
     if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl, S.acc_w);
     __syncthreads();
     const Ctrl *C = &S.ctrl;
-    if (!uni(C->active)) return;
+    if (!uni(C->active)) {
+        if (uni(C->copy_now)) copy_result_tile<TX, TY, NT>(I, C, decode_tile<TX, TY>(I, tl));
+        return;
+    }
     TileCoord tc = decode_tile<TX, TY>(I, tl);
     tc.x0 = uni(tc.x0); tc.y0 = uni(tc.y0); tc.z0 = uni(tc.z0); tc.z1 = uni(tc.z1);
 

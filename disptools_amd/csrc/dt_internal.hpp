@@ -57,6 +57,8 @@ struct Ctrl {
     int acc_prev;    // slot holding the previous pass's sums
     int acc_zero;    // slot the instance's first tile clears during this pass
     int evaluated;   // a trial has been evaluated since the state was last advanced
+    int copy_now;    // this pass copies the result into buf[0] (the pass that ended the solve)
+    int result_copied;   // the result has been copied into buf[0] by a pass
     // ---- controller state ----
     int phase;
     int base;        // accepted iterate

@@ -285,6 +285,8 @@ Ctrl initial_ctrl(int algorithm, const dt_params &p)
     c.acc_zero = 1;
     c.acc_prev = 2;
     c.evaluated = 0;
+    c.copy_now = 0;
+    c.result_copied = 0;
     return c;
 }
 
@@ -748,11 +750,12 @@ static int solve_device(int algorithm, int nx, int ny, int nz, float dx, float d
         rc = enqueue(chunk);
     }
     const bool timed = hipEventRecord(kev[1], st) == hipSuccess;
-    // result into the caller's field behind the passes (and the final control
-    // block, unless the device published it)
+    // the pass that ended the solve has copied the result into the caller's
+    // field (copy_result_tile); the final control block comes from the device
+    // unless the device published it
     Ctrl *fin_copies = (Ctrl *)(A->stage + offsetof(SolveHead, ctrl));
-    hipError_t se = rc ? hipSuccess : launch_take_result(d_inst, n, st);
-    if (se == hipSuccess && !rc && !published)
+    hipError_t se = hipSuccess;
+    if (!rc && !published)
         se = hipMemcpyAsync(fin_copies, h.ctrl, 2 * sizeof(Ctrl), hipMemcpyDeviceToHost, st);
     const hipError_t ss = hipStreamSynchronize(st);
     if (se == hipSuccess) se = ss;
