@@ -101,9 +101,25 @@ struct SumsWave {
     unsigned flags;
 };
 
-// Adds a block's sums into its copy of accumulator slot `slot`: wave sums by
-// shuffles, the waves' sums in LDS, then one thread's atomics (relaxed,
-// agent scope: the pass that reads them starts after this kernel has ended).
+// One thread adds a block's reduced sums into the block's copy of
+// accumulator slot `slot` (relaxed, agent scope: the pass that reads them
+// starts after this kernel has ended).
+__device__ __forceinline__ void commit_block_sums(const Inst &I, int slot, u128 e, u128 g, float mx, unsigned flags)
+{
+    Acc *a = I.acc + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned long long le = (unsigned long long)(unsigned)(e >> (32 * k));
+        const unsigned long long lg = (unsigned long long)(unsigned)(g >> (32 * k));
+        if (le) __hip_atomic_fetch_add(&a->e[k], le, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lg) __hip_atomic_fetch_add(&a->g[k], lg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (mx > 0.0f) __hip_atomic_fetch_max(&a->mx, __float_as_uint(mx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (flags) __hip_atomic_fetch_or(&a->flags, flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Adds a block's sums, held in registers, into its accumulator copy: wave
+// sums by shuffles, the waves' sums in LDS, then one thread's atomics.
 template <int NT>
 __device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx, unsigned flags, SumsWave *red)
 {
@@ -133,17 +149,47 @@ __device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx,
             m = fmaxf(m, red[i].mx);
             f |= red[i].flags;
         }
-        Acc *a = I.acc + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned long long le = (unsigned long long)(unsigned)(se >> (32 * k));
-            const unsigned long long lg = (unsigned long long)(unsigned)(sg >> (32 * k));
-            if (le) __hip_atomic_fetch_add(&a->e[k], le, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (lg) __hip_atomic_fetch_add(&a->g[k], lg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (m > 0.0f) __hip_atomic_fetch_max(&a->mx, __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (f) __hip_atomic_fetch_or(&a->flags, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        commit_block_sums(I, slot, se, sg, m, f);
     }
+}
+
+// Adds a block's sums into its accumulator copy when the exact sums sit in
+// LDS, one pair per tile thread (accE/accG[0 .. NOWN)): every wave leaves its
+// max and flags in `red`, then after one barrier wave 0 alone sums the LDS
+// pairs (NOWN / 64 per lane), reduces by shuffles and commits; the other
+// waves are done at the barrier.  The sums are integers, so the order of
+// the additions does not change them.
+template <int NT, int NOWN>
+__device__ void add_pass_sums_lds(const Inst &I, int slot, const u128 *accE, const u128 *accG, float mx,
+                                  unsigned flags, SumsWave *red)
+{
+    constexpr int NW = NT / 64;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) flags |= (unsigned)__shfl_xor((int)flags, o, 64);
+    mx = wave_max(mx);
+    const int lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[threadIdx.x >> 6].mx = mx;
+        red[threadIdx.x >> 6].flags = flags;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    u128 e = 0, g = 0;
+#pragma unroll 4
+    for (int i = lane; i < NOWN; i += 64) {
+        e += accE[i];
+        g += accG[i];
+    }
+    mx = lane < NW ? red[lane].mx : 0.0f;
+    flags = lane < NW ? red[lane].flags : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        e += shfl_xor_u128(e, o);
+        g += shfl_xor_u128(g, o);
+        flags |= (unsigned)__shfl_xor((int)flags, o, 64);
+    }
+    mx = wave_max(mx);
+    if (lane == 0) commit_block_sums(I, slot, e, g, mx, flags);
 }
 
 // 128-bit fixed-point value (quantum 2^-64) as a double; deterministic.

@@ -30,7 +30,10 @@
 //    once), and target / mask alternate between two register sets (the loop
 //    body is two steps).  A loaded register copied at the back edge makes the
 //    step wait for the load it has just issued, which turns the one-plane
-//    prefetch into a full memory round trip per step.
+//    prefetch into a full memory round trip per step;
+//  * the tile threads' exact sums stay in LDS to the end of the pass; after
+//    one barrier wave 0 alone folds them and commits the block's sums, and
+//    the other waves leave.
 #include <type_traits>
 
 #include "dt_device.hpp"
@@ -474,9 +477,7 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, in
     else
         stream_pass_body<CT, TX, TY, false>(S, I, C, tc, flags, mx);
 
-    const bool own = threadIdx.x < StreamGeom<TX, TY>::NOWN;
-    add_pass_sums<NT>(I, uni(C->acc_cur), own ? S.accE[threadIdx.x] : (u128)0, own ? S.accG[threadIdx.x] : (u128)0,
-                      mx, flags, S.red);
+    add_pass_sums_lds<NT, StreamGeom<TX, TY>::NOWN>(I, uni(C->acc_cur), S.accE, S.accG, mx, flags, S.red);
 }
 
 // One (precision, tile) variant per compilation of this file: the register
