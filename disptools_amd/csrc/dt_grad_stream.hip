@@ -18,7 +18,10 @@
 //    so the z-neighbours of its point come from registers;
 //  * LDS holds a ring of three trial planes (halo 3) and two planes of
 //    W_{a,x} / W_{a,y} (halo 1 along their own axis), indexed by plane
-//    parity: one barrier per plane separates every write from its reads;
+//    parity: one barrier per plane separates every write from its reads.
+//    A point's three values share one 16-byte cell, so a stencil neighbour
+//    is one 96-bit LDS read; the W planes alternate by a step's place in the
+//    two-step trip, so their offsets are constants;
 //  * the rest of the halo (rings 2 and 3, ring-1 corners) is spread over all
 //    threads;
 //  * trips whose two planes lie inside the z-chunk and the volume and off
@@ -51,16 +54,24 @@ struct StreamGeom {
     static constexpr int NOTHER = PL - NOWN - NRING;              // rings 2-3 and ring-1 corners
     static constexpr int SX = (3 * NOTHER + NT - 1) / NT;         // halo slots per thread
     static constexpr int SLOTS = 3 + SX;
+    static constexpr int WXS = TY * (TX + 2) + 1;                 // cells of a W_{a,x} plane + its sink
+    static constexpr int WYS = (TY + 2) * TX + 1;                 // cells of a W_{a,y} plane + its sink
     static_assert(NOWN % 64 == 0, "tile threads fill whole waves");
     static_assert(4 * HX + 4 * (HY - 4) + 4 == NOTHER, "halo decomposition");
 };
 
+// One LDS cell: the three values of a point (field components or a
+// relation axis' three weighted cofactors), 16-byte aligned, moved by one
+// 96-bit LDS access.
+typedef float f3 __attribute__((ext_vector_type(3)));
+static_assert(sizeof(f3) == 16, "an LDS cell is 16 bytes");
+
 template <int TX, int TY>
 struct StreamSmem {
     using G = StreamGeom<TX, TY>;
-    float xs[3][3][G::PL];            // ring of three trial planes, halo 3
-    float wx[2][3][TY][TX + 2];       // W_{a,x} by parity, x-halo 1
-    float wy[2][3][TY + 2][TX];       // W_{a,y} by parity, y-halo 1
+    f3 xs[3][G::PL];                  // ring of three trial planes, halo 3 (components x, y, z of a point)
+    f3 wx[2][G::WXS];                 // W_{a,x}, a = 0..2, [TY][TX + 2] (x-halo 1) + sink, two planes
+    f3 wy[2][G::WYS];                 // W_{a,y}, a = 0..2, [TY + 2][TX] (y-halo 1) + sink, two planes
     u128 accE[G::NOWN], accG[G::NOWN];   // exact sums of the tile threads (kept here, not in registers)
     SumsWave red[G::NT / 64];
     Ctrl ctrl;                        // this pass's control state (pass_begin)
@@ -195,11 +206,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
     const ColCoef K = column_coefficients(P, gx, gy);
     const int pi = (ly + 3) * HX + (lx + 3);
+    // W cells of the primary point; a ring point's other relation axis: the sink
+    const int wxc = (own || what == 1) ? ly * (TX + 2) + lx + 1 : G::WXS - 1;
+    const int wyc = (own || what == 2) ? (ly + 1) * TX + lx : G::WYS - 1;
 
     // halo slots: a share of rings 2-3 and the ring-1 corners (slots past the
     // last point repeat points from the start)
     unsigned xv[SX];   // voffset
-    int xd[SX];        // LDS index in a plane triple
+    int xd[SX];        // float index in a plane of cells
     unsigned xok = 0;
 #pragma unroll
     for (int j = 0; j < SX; ++j) {
@@ -210,7 +224,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         const int x = tc.x0 - 3 + xx, y = tc.y0 - 3 + yy;
         const bool in = x >= 0 && x < nx && y >= 0 && y < ny;
         xv[j] = in ? 4u * ((unsigned)c * n + (unsigned)(y * nx + x)) : 0u;
-        xd[j] = c * PL + yy * HX + xx;
+        xd[j] = 4 * (yy * HX + xx) + c;
         xok |= (in ? 1u : 0u) << j;
     }
 
@@ -244,9 +258,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         }
     };
     auto commit = [&](int slot, const Val &o) {
-        float *dst = &S.xs[slot][0][0];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) dst[c * PL + pi] = o.v[c];
+        S.xs[slot][pi] = f3{o.v[0], o.v[1], o.v[2]};
+        float *dst = (float *)&S.xs[slot][0];
 #pragma unroll
         for (int j = 0; j < SX; ++j) dst[xd[j]] = o.v[3 + j];
     };
@@ -321,9 +334,11 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     };
 
     // step zz: plane zz+2 joins the window and the LDS ring, stage A on plane
-    // zz (ring slot sa), stage B on plane zz-1
-    auto step = [&](auto fast, int zz, int sa, TM &cur, TM &nxt) {
+    // zz (ring slot sa), stage B on plane zz-1.  The W planes alternate by the
+    // step's place in its trip: PZ receives plane zz, 1 - PZ holds plane zz-1.
+    auto step = [&](auto fast, auto pz, int zz, int sa, TM &cur, TM &nxt) {
         constexpr bool FAST = decltype(fast)::value;
+        constexpr int PZ = decltype(pz)::value;
         // The loads of plane zz+2 and of this plane's target / mask were issued
         // one step ago.  An empty asm that takes them here keeps the compiler
         // from hoisting their first use above the barrier that ended the last
@@ -352,32 +367,23 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         // volume too: nothing reads their W, their error is not counted)
         CT wz_p[3] = {0, 0, 0};
         if (FAST || (zz >= 0 && zz < nz)) {
-            const float *R0 = &S.xs[sa][0][0] + pi;
+            const f3 *R = &S.xs[sa][pi];
+            const f3 xm2 = R[-2], xm1 = R[-1], xp1 = R[1], xp2 = R[2];
+            const f3 ym2 = R[-2 * HX], ym1 = R[-HX], yp1 = R[HX], yp2 = R[2 * HX];
             float sm[3][3][4];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float *R = R0 + c * PL;
-                sm[c][0][0] = R[-2]; sm[c][0][1] = R[-1]; sm[c][0][2] = R[1]; sm[c][0][3] = R[2];
-                sm[c][1][0] = R[-2 * HX]; sm[c][1][1] = R[-HX]; sm[c][1][2] = R[HX]; sm[c][1][3] = R[2 * HX];
+                sm[c][0][0] = xm2[c]; sm[c][0][1] = xm1[c]; sm[c][0][2] = xp1[c]; sm[c][0][3] = xp2[c];
+                sm[c][1][0] = ym2[c]; sm[c][1][1] = ym1[c]; sm[c][1][2] = yp1[c]; sm[c][1][3] = yp2[c];
                 sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
             }
             const bool count = own && pin && (FAST || (zz >= z0 && zz < z1));
             CT W[9];
             stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W,
                                FAST || (zz >= 2 && zz <= nz - 3));
-            const int pz = zz & 1;
-            if (own) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) S.wx[pz][a][ly][lx + 1] = (float)W[a];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) S.wy[pz][a][ly + 1][lx] = (float)W[3 + a];
-            } else {
-                // a ring point stores its relation axis' three values
-                float *wdst = what == 1 ? &S.wx[pz][0][ly][lx + 1] : &S.wy[pz][0][ly + 1][lx];
-                const int wstride = what == 1 ? TY * (TX + 2) : (TY + 2) * TX;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) wdst[a * wstride] = (float)(what == 1 ? W[a] : W[3 + a]);
-            }
+            // (a ring point's other relation axis goes to the sink cell)
+            S.wx[PZ][wxc] = f3{(float)W[0], (float)W[1], (float)W[2]};
+            S.wy[PZ][wyc] = f3{(float)W[3], (float)W[4], (float)W[5]};
 #pragma unroll
             for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
         }
@@ -388,15 +394,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
         if (own && (FAST || (zb >= z0 && zb < z1))) {
-            const int pb = zb & 1;
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
             if (FAST || (zb >= 1 && zb <= nz - 2)) {
-                const CT dxx = (CT)S.wx[pb][0][ly][lx] - (CT)S.wx[pb][0][ly][lx + 2];
-                const CT dyx = (CT)S.wx[pb][1][ly][lx] - (CT)S.wx[pb][1][ly][lx + 2];
-                const CT dzx = (CT)S.wx[pb][2][ly][lx] - (CT)S.wx[pb][2][ly][lx + 2];
-                const CT dxy = (CT)S.wy[pb][0][ly][lx] - (CT)S.wy[pb][0][ly + 2][lx];
-                const CT dyy = (CT)S.wy[pb][1][ly][lx] - (CT)S.wy[pb][1][ly + 2][lx];
-                const CT dzy = (CT)S.wy[pb][2][ly][lx] - (CT)S.wy[pb][2][ly + 2][lx];
+                const f3 xl = S.wx[1 - PZ][wxc - 1], xr = S.wx[1 - PZ][wxc + 1];
+                const f3 yl = S.wy[1 - PZ][wyc - TX], yr = S.wy[1 - PZ][wyc + TX];
+                const CT dxx = (CT)xl[0] - (CT)xr[0];
+                const CT dyx = (CT)xl[1] - (CT)xr[1];
+                const CT dzx = (CT)xl[2] - (CT)xr[2];
+                const CT dxy = (CT)yl[0] - (CT)yr[0];
+                const CT dyy = (CT)yl[1] - (CT)yr[1];
+                const CT dzy = (CT)yl[2] - (CT)yr[2];
                 // the gradient is zero on the outer face (and unused outside the volume)
                 f0 = col_interior ? (float)(cix * dxx - ciy * dxy + ciz * (wz_m[0] - wz_p[0])) : 0.0f;
                 f1 = col_interior ? (float)(-cix * dyx + ciy * dyy - ciz * (wz_m[1] - wz_p[1])) : 0.0f;
@@ -429,14 +436,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 #pragma unroll 1
     for (int zz = zs; zz < z1; zz += 2, ++t) {
         if (t >= t_lo && t <= t_hi) {
-            step(std::true_type{}, zz, sa, tma, tmb);
+            step(std::true_type{}, std::integral_constant<int, 0>{}, zz, sa, tma, tmb);
             sa = sa == 2 ? 0 : sa + 1;
-            step(std::true_type{}, zz + 1, sa, tmb, tma);
+            step(std::true_type{}, std::integral_constant<int, 1>{}, zz + 1, sa, tmb, tma);
             sa = sa == 2 ? 0 : sa + 1;
         } else {
-            step(std::false_type{}, zz, sa, tma, tmb);
+            step(std::false_type{}, std::integral_constant<int, 0>{}, zz, sa, tma, tmb);
             sa = sa == 2 ? 0 : sa + 1;
-            step(std::false_type{}, zz + 1, sa, tmb, tma);
+            step(std::false_type{}, std::integral_constant<int, 1>{}, zz + 1, sa, tmb, tma);
             sa = sa == 2 ? 0 : sa + 1;
         }
     }
