@@ -110,6 +110,8 @@ def parse():
     ap.add_argument("--qps", type=float, default=80.0)
     ap.add_argument("--graphs", type=int, default=0, help="engine replays chunks from hipGraphs")
     ap.add_argument("--slots", type=int, default=0, help="engine slots (instances solved at once; 0: 256)")
+    ap.add_argument("--lane-variant", type=int, default=-1,
+                    help="gradient column tile of the engine's lane (0-4; -1: the engine's choice)")
     ap.add_argument("--tz", type=int, default=0,
                     help="engine z-chunk policy (0: re-planned per set, -1: lone-solve, -2: <= 16, > 0: fixed)")
     ap.add_argument("--serve-dump", default="", help="write per-request records of the serving run (JSON)")
@@ -171,7 +173,7 @@ def _iterations(info):
     return info["iterations"] - (1 if info["reason"] == 6 else 0)
 
 
-def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160), tz=0):
+def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160), tz=0, lane_variant=-1):
     """64 volumes of uneven sizes (each axis U{48..160}, spacing 1): solved one
     dt_solve_device (matching: dt_matching_device) call after the other (the
     plain one-volume-per-launch solve) and all at once by the batch engine,
@@ -211,7 +213,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160), tz
     vols[big][2].zero_()
     vols[0][2].zero_()
     plain(0)
-    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz, variant=lane_variant) as eng:
         vols[0][2].zero_()
         T, M, F, sp = vols[0]
         eng.submit(algorithm, T.shape[2], T.shape[1], T.shape[0], sp, T.data_ptr(), M.data_ptr(),
@@ -233,7 +235,7 @@ def batch_bench(count, it_max, graphs, algorithm="gradient", sides=(48, 160), tz
     for v in vols:
         v[2].zero_()
     torch.cuda.synchronize()
-    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, tz=tz, variant=lane_variant) as eng:
         t0 = time.perf_counter()
         tickets = {}
         for i, (T, M, F, sp) in enumerate(vols):
@@ -279,7 +281,7 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0.0, tz=0):
+def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0.0, tz=0, lane_variant=-1):
     """Open-loop serving stream: Poisson arrivals at `qps`, cube side
     log-normal (median 96, sigma_ln 0.35, clipped to [32, 256]), 80 % gradient
     and 20 % greedy requests, targets and masks of the 128^3 workload's
@@ -364,7 +366,7 @@ def serve_bench(count, qps, it_max, graphs, seed=7, slots=0, dump="", matching=0
     vi_b = 0
     for d in data:
         d[2].zero_()
-    with native.Engine(use_graphs=bool(graphs), stream=sh, max_slots=slots, tz=tz) as eng:
+    with native.Engine(use_graphs=bool(graphs), stream=sh, max_slots=slots, tz=tz, variant=lane_variant) as eng:
         queue = collections.deque()
         tickets = {}
         nxt = 0
@@ -587,11 +589,13 @@ def main():
             raise RuntimeError("no HIP device")
         if args.mode == "batch":
             sides = tuple(int(v) for v in args.batch_sides.split(","))
-            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides, args.tz)))
+            print(json.dumps(batch_bench(args.batch or 64, args.it_max, args.graphs, args.algorithm, sides, args.tz,
+                                          args.lane_variant)))
         else:
             print(json.dumps(serve_bench(args.serve or 100, args.qps, args.it_max, args.graphs,
                                          matching=args.serve_matching,
-                                         slots=args.slots, dump=args.serve_dump, tz=args.tz)))
+                                         slots=args.slots, dump=args.serve_dump, tz=args.tz,
+                                         lane_variant=args.lane_variant)))
         return
 
     import numpy as np
@@ -665,9 +669,10 @@ def main():
     avg_launch_s = sum(i["kernel_seconds"] for i in infos) / launches
     achieved = alg_bytes / launches / avg_launch_s / 1e9
 
-    batch = batch_bench(args.batch, args.it_max, args.graphs, tz=args.tz) if args.batch > 0 else None
+    batch = (batch_bench(args.batch, args.it_max, args.graphs, tz=args.tz, lane_variant=args.lane_variant)
+             if args.batch > 0 else None)
     serving = (serve_bench(args.serve, args.qps, args.it_max, args.graphs, slots=args.slots,
-                           dump=args.serve_dump, tz=args.tz)
+                           dump=args.serve_dump, tz=args.tz, lane_variant=args.lane_variant)
                if args.serve > 0 else None)
 
     cores, cores_source = physical_cores()

@@ -17,6 +17,7 @@ hipError_t launch_regularise(float *J, long long n, float eps, hipStream_t st);
 // variant gives the same results.
 constexpr int GRAD_VARIANTS = 5;
 constexpr int DEFAULT_GRAD_VARIANT = 4;   // 64 x 12
+constexpr int ENGINE_GRAD_VARIANT = 3;    // 32 x 24 (engine lanes)
 inline void grad_variant_dims(int v, int &tx, int &ty)
 {
     switch (v) {

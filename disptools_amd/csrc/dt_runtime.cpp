@@ -349,8 +349,9 @@ int choose_tz(int nx, int ny, int nz, int tx, int ty, int cus, int resident, int
 
 // Default pass variant (DEFAULT_GRAD_VARIANT): 64 x 12 column tiles with the
 // tile ring in waves of its own (measured fastest at 128^3 among the compiled
-// variants).  Engine lanes use it; single solves start from it and switch
-// tile by model_grad_variant; DISPTOOLS_GRAD_VARIANT overrides both.
+// variants).  Single solves start from it and switch tile by
+// model_grad_variant; engine lanes use ENGINE_GRAD_VARIANT;
+// DISPTOOLS_GRAD_VARIANT overrides both.
 int variant_from_env()
 {
     const char *s = std::getenv("DISPTOOLS_GRAD_VARIANT");

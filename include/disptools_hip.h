@@ -309,7 +309,7 @@ typedef struct dt_engine_config {
     int max_slots;       /* instances on the device at once (0: 256; at most 4096)       */
     int chunk;           /* passes enqueued per chunk (0: 2)                               */
     int use_graphs;      /* 1: each chunk is replayed from a captured hipGraph             */
-    int variant;         /* gradient pass variant (-1: the library default)                */
+    int variant;         /* gradient pass variant (-1: 32 x 24 column tiles)               */
     int tz;              /* z-chunk depth in planes: 0 = re-planned for the set of
                             instances whenever it changes (balanced chunks, the bound
                             that fills the CUs best), -1 = as a lone dt_solve_device
