@@ -45,6 +45,12 @@ __device__ __forceinline__ T *uni(T *p)
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
+// One LDS cell: the three values of a point (field components, or a relation
+// axis' three weighted cofactors), 16-byte aligned, moved by one 96-bit LDS
+// access.
+typedef float f3 __attribute__((ext_vector_type(3)));
+static_assert(sizeof(f3) == 16, "an LDS cell is 16 bytes");
+
 // ---------------------------------------------------------------------------
 // Exact sums (the Acc layout: 128-bit fixed point, quantum 2^-64, 32-bit limbs)
 // ---------------------------------------------------------------------------

@@ -60,12 +60,6 @@ struct StreamGeom {
     static_assert(4 * HX + 4 * (HY - 4) + 4 == NOTHER, "halo decomposition");
 };
 
-// One LDS cell: the three values of a point (field components or a
-// relation axis' three weighted cofactors), 16-byte aligned, moved by one
-// 96-bit LDS access.
-typedef float f3 __attribute__((ext_vector_type(3)));
-static_assert(sizeof(f3) == 16, "an LDS cell is 16 bytes");
-
 template <int TX, int TY>
 struct StreamSmem {
     using G = StreamGeom<TX, TY>;
