@@ -35,7 +35,10 @@ HIP_FLAGS = [
 
 # Per-source contraction: the reference is ISO C99 without contraction, so the
 # standalone jacobian keeps every multiply and add separately rounded (it is
-# bit-exact with the reference); the solver pass is FP32 with FMA contraction.
+# bit-exact with the reference).  The gradient pass writes each of its fused
+# multiply-adds out and is compiled with contraction off, so every tile
+# variant rounds every voxel the same way (the compiler could otherwise fuse
+# a*b - c*d either way, differently per instantiation).
 HIP_SOURCES = [("dt_kernels.hip", "-ffp-contract=off"),
                ("dt_jacobian.hip", "-ffp-contract=off"),
                ("dt_greedy_stream.hip", "-ffp-contract=fast-honor-pragmas"),
@@ -58,7 +61,7 @@ def _stream_objects():
     out = []
     for ct, tx, ty, w in STREAM_VARIANTS:
         name = "%s_%dx%d_w%d" % ("f32" if ct == "float" else "f64", tx, ty, w)
-        out.append(("dt_grad_stream.hip", "-ffp-contract=fast",
+        out.append(("dt_grad_stream.hip", "-ffp-contract=off",
                     ["-DDT_STREAM_CT=%s" % ct, "-DDT_STREAM_TX=%d" % tx, "-DDT_STREAM_TY=%d" % ty,
                      "-DDT_STREAM_MINW=%d" % w, "-DDT_STREAM_NAME=%s" % name], "dt_grad_stream_%s.o" % name))
     return out

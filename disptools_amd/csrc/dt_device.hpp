@@ -78,26 +78,72 @@ __device__ __forceinline__ void fix_add(u128 &acc, double d, unsigned &flags, un
     flags |= (is_huge ? huge : 0u) | (is_nan ? nan : 0u);
 }
 
-__device__ __forceinline__ u128 shfl_xor_u128(u128 v, int o)
+// x of the lane a row-local DPP control selects (the whole wave active).
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_u32(unsigned x)
 {
-    unsigned w[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) w[k] = (unsigned)__shfl_xor((int)(unsigned)(v >> (32 * k)), o, 64);
-    return (u128)w[0] | ((u128)w[1] << 32) | ((u128)w[2] << 64) | ((u128)w[3] << 96);
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x)
+{
+    return (unsigned long long)dpp_u32<CTRL>((unsigned)x) |
+           ((unsigned long long)dpp_u32<CTRL>((unsigned)(x >> 32)) << 32);
+}
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long x, int lane)
+{
+    return (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, lane) |
+           ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), lane) << 32);
 }
 
-__device__ __forceinline__ float wave_max(float v)
+template <int CTRL>
+__device__ __forceinline__ u128 dpp_u128(u128 x)
 {
+    u128 r = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    for (int k = 0; k < 4; ++k) r |= (u128)dpp_u32<CTRL>((unsigned)(x >> (32 * k))) << (32 * k);
+    return r;
+}
+__device__ __forceinline__ u128 readlane_u128(u128 x, int lane)
+{
+    u128 r = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        r |= (u128)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> (32 * k)), lane) << (32 * k);
+    return r;
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+// Wave reductions (the whole wave active; results wave-uniform).  Four
+// row-local DPP steps -- lane xor 1, lane xor 2, mirror within 8, mirror
+// within 16 -- leave every lane of a row of 16 holding the row's result; the
+// four rows' results are then read into scalar registers and combined.
+// (Cross-lane permutes through the LDS cost a round trip per step.)
+__device__ __forceinline__ u128 wave_sum_u128(u128 v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_u128<0xB1>(v);
+    v += dpp_u128<0x4E>(v);
+    v += dpp_u128<0x141>(v);
+    v += dpp_u128<0x140>(v);
+    return readlane_u128(v, 0) + readlane_u128(v, 16) + readlane_u128(v, 32) + readlane_u128(v, 48);
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
+{
+    v = max(v, dpp_u32<0xB1>(v));
+    v = max(v, dpp_u32<0x4E>(v));
+    v = max(v, dpp_u32<0x141>(v));
+    v = max(v, dpp_u32<0x140>(v));
+    const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)v, 0), b = (unsigned)__builtin_amdgcn_readlane((int)v, 16);
+    const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)v, 32), d = (unsigned)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a, b), max(c, d));
+}
+__device__ __forceinline__ unsigned wave_or_u32(unsigned v)
+{
+    v |= dpp_u32<0xB1>(v);
+    v |= dpp_u32<0x4E>(v);
+    v |= dpp_u32<0x141>(v);
+    v |= dpp_u32<0x140>(v);
+    return (unsigned)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                      __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
 }
 
 // LDS scratch of the end-of-pass reduction: one entry per wave.
@@ -125,57 +171,54 @@ __device__ __forceinline__ void commit_block_sums(const Inst &I, int slot, u128 
 }
 
 // Adds a block's sums, held in registers, into its accumulator copy: wave
-// sums by shuffles, the waves' sums in LDS, then one thread's atomics.
+// reductions, the waves' results in LDS, then wave 0 reduces those and one
+// thread commits.  mx >= 0, so its bits order as unsigned integers.
 template <int NT>
 __device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx, unsigned flags, SumsWave *red)
 {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        e += shfl_xor_u128(e, o);
-        g += shfl_xor_u128(g, o);
-        flags |= (unsigned)__shfl_xor((int)flags, o, 64);
-    }
-    mx = wave_max(mx);
+    static_assert(NW <= 64, "one wave's results per lane");
+    e = wave_sum_u128(e);
+    g = wave_sum_u128(g);
+    const unsigned m = wave_max_u32(__float_as_uint(mx));
+    flags = wave_or_u32(flags);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) {
         red[w].e = e;
         red[w].g = g;
-        red[w].mx = mx;
+        red[w].mx = __uint_as_float(m);
         red[w].flags = flags;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        u128 se = 0, sg = 0;
-        float m = 0.0f;
-        unsigned f = 0;
-        for (int i = 0; i < NW; ++i) {
-            se += red[i].e;
-            sg += red[i].g;
-            m = fmaxf(m, red[i].mx);
-            f |= red[i].flags;
-        }
-        commit_block_sums(I, slot, se, sg, m, f);
-    }
+    if (threadIdx.x >= 64) return;
+    u128 se = lane < NW ? red[lane].e : (u128)0;
+    u128 sg = lane < NW ? red[lane].g : (u128)0;
+    unsigned sm = lane < NW ? __float_as_uint(red[lane].mx) : 0u;
+    unsigned sf = lane < NW ? red[lane].flags : 0u;
+    se = wave_sum_u128(se);
+    sg = wave_sum_u128(sg);
+    sm = wave_max_u32(sm);
+    sf = wave_or_u32(sf);
+    if (lane == 0) commit_block_sums(I, slot, se, sg, __uint_as_float(sm), sf);
 }
 
 // Adds a block's sums into its accumulator copy when the exact sums sit in
 // LDS, one pair per tile thread (accE/accG[0 .. NOWN)): every wave leaves its
 // max and flags in `red`, then after one barrier wave 0 alone sums the LDS
-// pairs (NOWN / 64 per lane), reduces by shuffles and commits; the other
-// waves are done at the barrier.  The sums are integers, so the order of
+// pairs (NOWN / 64 per lane), reduces across its lanes and commits; the
+// other waves are done at the barrier.  The sums are integers, so the order of
 // the additions does not change them.
 template <int NT, int NOWN>
 __device__ void add_pass_sums_lds(const Inst &I, int slot, const u128 *accE, const u128 *accG, float mx,
                                   unsigned flags, SumsWave *red)
 {
     constexpr int NW = NT / 64;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) flags |= (unsigned)__shfl_xor((int)flags, o, 64);
-    mx = wave_max(mx);
+    static_assert(NW <= 64, "one wave's results per lane");
+    const unsigned m = wave_max_u32(__float_as_uint(mx));
+    flags = wave_or_u32(flags);
     const int lane = threadIdx.x & 63;
     if (lane == 0) {
-        red[threadIdx.x >> 6].mx = mx;
+        red[threadIdx.x >> 6].mx = __uint_as_float(m);
         red[threadIdx.x >> 6].flags = flags;
     }
     __syncthreads();
@@ -186,16 +229,13 @@ __device__ void add_pass_sums_lds(const Inst &I, int slot, const u128 *accE, con
         e += accE[i];
         g += accG[i];
     }
-    mx = lane < NW ? red[lane].mx : 0.0f;
-    flags = lane < NW ? red[lane].flags : 0u;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        e += shfl_xor_u128(e, o);
-        g += shfl_xor_u128(g, o);
-        flags |= (unsigned)__shfl_xor((int)flags, o, 64);
-    }
-    mx = wave_max(mx);
-    if (lane == 0) commit_block_sums(I, slot, e, g, mx, flags);
+    unsigned sm = lane < NW ? __float_as_uint(red[lane].mx) : 0u;
+    unsigned sf = lane < NW ? red[lane].flags : 0u;
+    e = wave_sum_u128(e);
+    g = wave_sum_u128(g);
+    sm = wave_max_u32(sm);
+    sf = wave_or_u32(sf);
+    if (lane == 0) commit_block_sums(I, slot, e, g, __uint_as_float(sm), sf);
 }
 
 // 128-bit fixed-point value (quantum 2^-64) as a double; deterministic.
@@ -215,21 +255,40 @@ struct PassSums {
 constexpr int ACC_WORDS = (int)(sizeof(Acc) / 8);
 static_assert(ACC_WORDS == 9, "Acc is nine 64-bit words");
 
-__device__ inline PassSums sums_of_slot(const unsigned long long *w)
+// One fold step of the accumulator records held one per lane: every lane adds
+// the words of the lane CTRL selects (limb sums add, max and flags combine).
+template <int CTRL>
+__device__ __forceinline__ void fold_records(unsigned long long (&w)[8], unsigned &mx, unsigned &fl)
 {
-    u128 e = 0, g = 0;
-    unsigned mxb = 0, flags = 0;
-    for (int k = 0; k < ACC_COPIES; ++k) {
-        const unsigned long long *a = w + k * ACC_WORDS;
 #pragma unroll
-        for (int l = 0; l < 4; ++l) {
-            e += (u128)a[l] << (32 * l);
-            g += (u128)a[4 + l] << (32 * l);
-        }
-        const unsigned m = (unsigned)a[8];
-        mxb = m > mxb ? m : mxb;
-        flags |= (unsigned)(a[8] >> 32);
+    for (int k = 0; k < 8; ++k) w[k] += dpp_u64<CTRL>(w[k]);
+    const unsigned m = dpp_u32<CTRL>(mx);
+    mx = m > mx ? m : mx;
+    fl |= dpp_u32<CTRL>(fl);
+}
+
+// The sums of accumulator slot `slot` when lane L < ACC_SLOTS * ACC_COPIES
+// holds record L (slot L / ACC_COPIES, copy L % ACC_COPIES): limb words w,
+// max bits mx and flags fl (run by a whole wave, `slot` uniform).  The eight
+// copies of a slot sit in consecutive lanes 8s .. 8s + 7 and are folded by
+// three row-local DPP steps (lane xor 1, lane xor 2, mirror within 8); the
+// slot's total is then read from lane 8s.  The words are sums of 32-bit limbs
+// (below 2^45 here), so the folds are exact and their order does not matter.
+__device__ inline PassSums sums_of_records(unsigned long long (&w)[8], unsigned mx, unsigned fl, int slot)
+{
+    static_assert(ACC_COPIES == 8, "a slot's copies fill one half-row");
+    fold_records<0xB1>(w, mx, fl);    // quad_perm [1, 0, 3, 2]
+    fold_records<0x4E>(w, mx, fl);    // quad_perm [2, 3, 0, 1]
+    fold_records<0x141>(w, mx, fl);   // row_half_mirror
+    const int src = slot * ACC_COPIES;
+    u128 e = 0, g = 0;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        e += (u128)readlane_u64(w[l], src) << (32 * l);
+        g += (u128)readlane_u64(w[4 + l], src) << (32 * l);
     }
+    const unsigned mxb = (unsigned)__builtin_amdgcn_readlane((int)mx, src);
+    const unsigned flags = (unsigned)__builtin_amdgcn_readlane((int)fl, src);
     PassSums s;
     s.E = (flags & ACC_E_NAN) ? (double)NAN : ((flags & ACC_E_HUGE) ? (double)INFINITY : fix_to_double(e));
     s.G = (flags & ACC_G_NAN) ? (double)NAN : ((flags & ACC_G_HUGE) ? (double)INFINITY : fix_to_double(g));
@@ -241,13 +300,25 @@ __device__ inline PassSums sums_of_slot(const unsigned long long *w)
 // Controllers (run at the start of the next pass, on a copy of the state)
 // ---------------------------------------------------------------------------
 
+// What the controllers read of an instance besides its control block: the
+// solver parameters, the trace and the host-coherent status words.  The
+// controller step gathers it with the control block in one round trip (a
+// field read where a branch first needs it costs a round trip of its own).
+struct CtlEnv {
+    dt_params p;
+    double *trace;                  // Inst::trace .. Inst::final_ctrl, in order
+    long long trace_cap;
+    unsigned long long *status;
+    Ctrl *final_ctrl;
+};
+
 // A log line (it, E, max error, eta); only the instance's first tile writes
 // the trace (`leader`), every block counts it.
-__device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, bool leader, double it, double E, double m,
+__device__ __forceinline__ void log_line(const CtlEnv &V, Ctrl *C, bool leader, double it, double E, double m,
                                          double eta)
 {
-    if (leader && I.trace && C->trace_len < I.trace_cap) {
-        double *t = I.trace + 4 * C->trace_len;
+    if (leader && V.trace && C->trace_len < V.trace_cap) {
+        double *t = V.trace + 4 * C->trace_len;
         t[0] = it; t[1] = E; t[2] = m; t[3] = eta;
     }
     C->trace_len++;
@@ -259,18 +330,18 @@ __device__ __forceinline__ void log_line(const Inst &I, Ctrl *C, bool leader, do
 // launches.  Once the solve is done the control block itself is published
 // first (when the instance has a host copy), ordered before the word by a
 // system-scope release.
-__device__ __forceinline__ void publish_status(const Inst &I, const Ctrl *C)
+__device__ __forceinline__ void publish_status(const CtlEnv &V, const Ctrl *C)
 {
-    if (!I.status) return;
+    if (!V.status) return;
     const bool done = C->phase == PH_DONE;
-    if (done && I.final_ctrl) {
+    if (done && V.final_ctrl) {
         const unsigned long long *src = (const unsigned long long *)C;
-        unsigned long long *dst = (unsigned long long *)I.final_ctrl;
+        unsigned long long *dst = (unsigned long long *)V.final_ctrl;
         for (int k = 0; k < (int)(sizeof(Ctrl) / 8); ++k)
             __hip_atomic_store(dst + k, src[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __atomic_thread_fence(__ATOMIC_RELEASE);
     }
-    __hip_atomic_store(I.status, ((unsigned long long)C->evaluations << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
+    __hip_atomic_store(V.status, ((unsigned long long)C->evaluations << 1) | (done ? 1ull : 0ull), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
@@ -318,9 +389,10 @@ __device__ __forceinline__ void start_interp(Ctrl *C, const dt_params &p, bool s
 // Gradient solver control flow (src/displacement_field_gradient.c:250-421).
 // E, m, G are the energy, masked max error and squared gradient norm of the
 // trial the pass evaluated.
-static __device__ void grad_controller(const Inst &I, Ctrl *C, bool leader, double E, float m, double G)
+static __device__ __forceinline__ void grad_controller(const CtlEnv &V, Ctrl *C, bool leader, double E, float m,
+                                                      double G)
 {
-    const dt_params &p = I.p;
+    const dt_params &p = V.p;
     C->evaluations++;
     // algorithmic traffic of the pass that was just evaluated
     C->field_io += (C->mode == TRIAL_INTERP ? 2 : 1) + (C->out_trial >= 0 ? 1 : 0) + (C->out_next >= 0 ? 1 : 0);
@@ -331,7 +403,7 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, bool leader, doub
         C->E0 = E; C->m0 = m;
         C->E_last = E;
         C->G_base = G;
-        log_line(I, C, leader, 0.0, E, m, C->eta);
+        log_line(V, C, leader, 0.0, E, m, C->eta);
         if (p.it_max <= 0) { finish(C, DT_STOP_MAX_ITERATIONS); return; }
         C->cand_buf = C->out_next;
         C->eta_c = C->eta_next;
@@ -370,7 +442,7 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, bool leader, doub
             start_interp(C, p, true);
             return;
         }
-        log_line(I, C, leader, (double)C->it, E, m, C->eta);
+        log_line(V, C, leader, (double)C->it, E, m, C->eta);
         if (!isnormal((float)E)) { finish(C, DT_STOP_ERROR_EXPLODED); return; }
         if (C->eta < p.iota) { finish(C, DT_STOP_ETA_SMALL); return; }
         if (1.0 - E / C->E_last < (double)p.theta) { finish(C, DT_STOP_NO_IMPROVEMENT); return; }
@@ -404,9 +476,10 @@ static __device__ void grad_controller(const Inst &I, Ctrl *C, bool leader, doub
 // Greedy solver control flow (src/displacement_field_greedy.c:149-238).
 // `step_io` is the field volumes a pass with the step moves (2: the base read,
 // the trial written); the evaluation of the initial guess reads one.
-static __device__ void greedy_controller(const Inst &I, Ctrl *C, bool leader, double E, float m, int step_io)
+static __device__ __forceinline__ void greedy_controller(const CtlEnv &V, Ctrl *C, bool leader, double E, float m,
+                                                        int step_io)
 {
-    const dt_params &p = I.p;
+    const dt_params &p = V.p;
     C->evaluations++;
     C->field_io += C->phase == PH_INIT ? 1 : step_io;
     const int e_written = C->e_new;
@@ -424,7 +497,7 @@ static __device__ void greedy_controller(const Inst &I, Ctrl *C, bool leader, do
         return;
     }
     // the trial is buf[g_new]; its error map is err[e_written]
-    log_line(I, C, leader, (double)C->it, E, m, C->eta);
+    log_line(V, C, leader, (double)C->it, E, m, C->eta);
     C->e_cur = e_written;                 // the next step reads the newest map (:165-181)
     C->e_new = 1 - e_written;
     if (!isnormal((float)E)) { finish(C, DT_STOP_ERROR_EXPLODED); return; }
@@ -451,78 +524,97 @@ static __device__ void greedy_controller(const Inst &I, Ctrl *C, bool leader, do
 // The start of a pass
 // ---------------------------------------------------------------------------
 
-// LDS of pass_begin: every accumulator slot of the instance, as words.
-constexpr int BEGIN_ACC_WORDS = ACC_SLOTS * ACC_COPIES * ACC_WORDS;
-
-// Run by wave 0 of every block before the block's work: copies the control
-// block of this pass's parity into `c` (the block's LDS copy) and, when it
-// records an evaluated trial, advances the controller (ALG: DT_GRADIENT or
-// DT_GREEDY) by the sums of that pass.  Every block computes the same state
-// from the same inputs; the instance's first tile (`leader`) stores it into
-// the other copy for the next pass, clears the accumulator slot the next pass
-// adds into and publishes the completion word.  On return `c` is the state
-// this pass works from (active == 0: the solve has ended, the pass does
-// nothing); the caller synchronises the block before reading it.
+// Run by wave 0 of every block before the block's work: reads the control
+// block of this pass's parity and, when it records an evaluated trial,
+// advances the controller (ALG: DT_GRADIENT or DT_GREEDY) by the sums of that
+// pass, then leaves the state in `c` (the block's LDS copy).  Every block
+// computes the same state from the same inputs; the instance's first tile
+// (`leader`) stores it into the other copy for the next pass, clears the
+// accumulator slot the next pass adds into and publishes the completion
+// word.  On return `c` is the state this pass works from (active == 0: the
+// solve has ended, the pass does nothing); the caller synchronises the block
+// before reading it.
 //
-// The lanes load the control words and every slot's sums at once (which slot
-// holds the previous pass's sums is in the control block itself), so the
-// whole step costs one memory round trip; lane 0 then runs the controller on
-// the LDS copy (in registers the control block would raise the register
-// count of the kernel).
+// The lanes load the control words and every slot's accumulator records at
+// once (which slot holds the previous pass's sums is in the control block
+// itself), so the whole step costs one memory round trip.  The control
+// block is gathered into wave-uniform registers (one word per lane, read
+// across lanes) and the controller runs there: a controller on an LDS copy
+// waits for one LDS round trip per field it touches.
 template <int ALG>
-__device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c, unsigned long long *acc_w)
+__device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
 {
     constexpr int W = (int)(sizeof(Ctrl) / 8);
-    constexpr int AR = (BEGIN_ACC_WORDS + 63) / 64;
-    static_assert(W <= 64, "one control word per lane");
+    constexpr int PW = (int)(sizeof(dt_params) / 8), VW = (int)(sizeof(CtlEnv) / 8);
+    constexpr int NREC = ACC_SLOTS * ACC_COPIES;
+    static_assert(W + VW <= 64 && NREC <= 64, "one word and one accumulator record per lane");
+    static_assert(sizeof(dt_params) % 8 == 0 && offsetof(CtlEnv, trace) == sizeof(dt_params), "CtlEnv layout");
+    static_assert(offsetof(Inst, final_ctrl) - offsetof(Inst, trace) == 24, "Inst::trace .. final_ctrl");
     const int lane = threadIdx.x & 63;
-    const unsigned long long *src = (const unsigned long long *)(I.ctrl + parity);
-    const unsigned long long *asrc = (const unsigned long long *)I.acc;
+    // the instance's control copies and accumulators, kept in registers for
+    // the leader's stores (the compiler would otherwise reload them from the
+    // instance record there, a round trip of its own)
+    Ctrl *ctrl = I.ctrl;
+    Acc *acc = I.acc;
+    asm volatile("" : "+s"(ctrl), "+s"(acc));
+    const unsigned long long *src = (const unsigned long long *)(ctrl + parity);
+    const unsigned long long *rec = (const unsigned long long *)(acc + (lane < NREC ? lane : 0));
+    // lane k < W: word k of the control block; W <= k < W + VW: word k - W
+    // of the CtlEnv (the parameters, then Inst::trace .. Inst::final_ctrl)
+    // (every lane loads from a valid address, so the loads need no branches)
+    const unsigned long long *wp = lane < W ? src + lane
+                                 : lane < W + PW ? (const unsigned long long *)&I.p + (lane - W)
+                                 : lane < W + VW ? (const unsigned long long *)&I.trace + (lane - W - PW)
+                                                 : src;
+    const unsigned long long cw = *wp;
+    unsigned long long aw[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) aw[k] = rec[k];
+    const unsigned long long a8 = rec[8];
+    const bool has_rec = lane < NREC;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) aw[k] = has_rec ? aw[k] : 0ull;
+    const unsigned amx = has_rec ? (unsigned)a8 : 0u;
+    const unsigned afl = has_rec ? (unsigned)(a8 >> 32) : 0u;
+    unsigned long long words[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) words[k] = readlane_u64(cw, k);
+    Ctrl s;
+    __builtin_memcpy(&s, words, sizeof(Ctrl));
+    unsigned long long vwords[VW];
+#pragma unroll
+    for (int k = 0; k < VW; ++k) vwords[k] = readlane_u64(cw, W + k);
+    CtlEnv V;
+    __builtin_memcpy(&V, vwords, sizeof(CtlEnv));
+    if (s.active && s.evaluated) {
+        const PassSums ps = sums_of_records(aw, amx, afl, s.acc_cur);
+        // (only lane 0 of the leader writes log lines)
+        if (ALG == DT_GRADIENT) grad_controller(V, &s, leader && lane == 0, ps.E, ps.m, ps.G);
+        else greedy_controller(V, &s, leader && lane == 0, ps.E, ps.m, 2);
+        const int prev = s.acc_prev;
+        s.acc_prev = s.acc_cur;
+        s.acc_cur = s.acc_zero;
+        s.acc_zero = prev;
+    }
+    s.evaluated = s.active;
+    // the pass in which the solve ends copies its result into buf[0]
+    // (once: the copy of the next pass records it as done)
+    s.copy_now = (!s.active && s.phase == PH_DONE && !s.result_copied) ? 1 : 0;
+    if (s.copy_now) s.result_copied = 1;
+    if (lane == 0) *c = s;
     unsigned long long *dst = (unsigned long long *)c;
-    unsigned long long cw = 0, aw[AR];
-    if (lane < W) cw = src[lane];
-#pragma unroll
-    for (int k = 0; k < AR; ++k) {
-        const int i = lane + 64 * k;
-        aw[k] = i < BEGIN_ACC_WORDS ? asrc[i] : 0ull;
-    }
-    if (lane < W) dst[lane] = cw;
-#pragma unroll
-    for (int k = 0; k < AR; ++k) {
-        const int i = lane + 64 * k;
-        if (i < BEGIN_ACC_WORDS) acc_w[i] = aw[k];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane == 0) {
-        if (c->active && c->evaluated) {
-            const PassSums s = sums_of_slot(acc_w + c->acc_cur * ACC_COPIES * ACC_WORDS);
-            if (ALG == DT_GRADIENT) grad_controller(I, c, leader, s.E, s.m, s.G);
-            else greedy_controller(I, c, leader, s.E, s.m, 2);
-            const int prev = c->acc_prev;
-            c->acc_prev = c->acc_cur;
-            c->acc_cur = c->acc_zero;
-            c->acc_zero = prev;
-        }
-        c->evaluated = c->active;
-        // the pass in which the solve ends copies its result into buf[0]
-        // (once: the copy of the next pass records it as done)
-        c->copy_now = (!c->active && c->phase == PH_DONE && !c->result_copied) ? 1 : 0;
-        if (c->copy_now) c->result_copied = 1;
-    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (leader) {
         // an ended solve stays ended at both parities
-        unsigned long long *next = (unsigned long long *)(I.ctrl + (parity ^ 1));
+        unsigned long long *next = (unsigned long long *)(ctrl + (parity ^ 1));
         if (lane < W) next[lane] = dst[lane];
-        if (c->active) {
-            unsigned long long *z = (unsigned long long *)(I.acc + (long long)c->acc_zero * ACC_COPIES);
+        if (s.active) {
+            unsigned long long *z = (unsigned long long *)(acc + (long long)s.acc_zero * ACC_COPIES);
             for (int i = lane; i < ACC_COPIES * ACC_WORDS; i += 64) z[i] = 0ull;
         }
-        if (lane == 0) publish_status(I, c);
+        if (lane == 0) publish_status(V, &s);
     }
 }
 

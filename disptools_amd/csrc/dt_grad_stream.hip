@@ -51,66 +51,71 @@ struct StreamGeom {
     static constexpr int NRING = 2 * TX + 2 * TY;                 // ring-1 points without corners
     static constexpr int NT = NOWN + (NRING + 63) / 64 * 64;      // block size
     static constexpr int HX = TX + 6, HY = TY + 6, PL = HX * HY;
-    static constexpr int NOTHER = PL - NOWN - NRING;              // rings 2-3 and ring-1 corners
+    // the rest of the stencils' reach: rings 2-3 along the tile's sides and
+    // 12 cells at the corners (the rings' outer corner cells are never read)
+    static constexpr int NOTHER = 4 * TX + 4 * TY + 12;
     static constexpr int SX = (3 * NOTHER + NT - 1) / NT;         // halo slots per thread
     static constexpr int SLOTS = 3 + SX;
-    static constexpr int WXS = TY * (TX + 2) + 1;                 // cells of a W_{a,x} plane + its sink
-    static constexpr int WYS = (TY + 2) * TX + 1;                 // cells of a W_{a,y} plane + its sink
+    static constexpr int WPL = (TY + 2) * HX;                     // cells of a W plane: rows -1 .. TY, pitch HX
     static_assert(NOWN % 64 == 0, "tile threads fill whole waves");
-    static_assert(4 * HX + 4 * (HY - 4) + 4 == NOTHER, "halo decomposition");
+    static_assert(NOTHER + 24 == PL - NOWN - NRING, "halo decomposition");
 };
 
 template <int TX, int TY>
 struct StreamSmem {
     using G = StreamGeom<TX, TY>;
-    f3 xs[3][G::PL];                  // ring of three trial planes, halo 3 (components x, y, z of a point)
-    f3 wx[2][G::WXS];                 // W_{a,x}, a = 0..2, [TY][TX + 2] (x-halo 1) + sink, two planes
-    f3 wy[2][G::WYS];                 // W_{a,y}, a = 0..2, [TY + 2][TX] (y-halo 1) + sink, two planes
     u128 accE[G::NOWN], accG[G::NOWN];   // exact sums of the tile threads (kept here, not in registers)
+    f3 xs[3][G::PL];                  // ring of three trial planes, halo 3 (components x, y, z of a point)
+    f3 w[2][2][G::WPL];               // [plane parity][relation axis x, y]: W_{a,x} / W_{a,y}, a = 0..2
     SumsWave red[G::NT / 64];
     Ctrl ctrl;                        // this pass's control state (pass_begin)
-    unsigned long long acc_w[BEGIN_ACC_WORDS];   // the instance's accumulators (pass_begin)
 };
 
-// (xx, yy) in the halo-3 plane of the r-th point of rings 2-3 and ring-1 corners.
+// (xx, yy) in the halo-3 plane of the r-th halo point a stencil reaches beyond
+// the tile and its ring (points of the tile and ring 1 without corners are
+// stencil centres; a centre reads +-1 and +-2 along x and along y): rows
+// -3, -2 above and TY + 1, TY + 2 below the tile, columns -3, -2 left and
+// TX + 1, TX + 2 right of it, and at each corner the three cells the ring
+// centres next to it reach (e.g. (-1, -1), (-2, -1), (-1, -2)).  Coordinates
+// are tile-relative; the plane index adds 3.
 template <int TX, int TY>
 __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 {
-    using G = StreamGeom<TX, TY>;
-    constexpr int HX = G::HX, HY = G::HY;
-    if (r < 2 * HX) {
-        yy = r / HX; xx = r - yy * HX;
-    } else if (r < 4 * HX) {
-        const int q = r - 2 * HX;
-        yy = HY - 2 + q / HX; xx = q % HX;
-    } else if (r < 4 * HX + 4 * (HY - 4)) {
-        const int q = r - 4 * HX;
-        yy = 2 + q / 4;
-        const int k = q & 3;
-        xx = k < 2 ? k : HX - 4 + k;
-    } else {
-        const int q = r - 4 * HX - 4 * (HY - 4);
-        xx = (q & 1) ? HX - 3 : 2;
-        yy = (q & 2) ? HY - 3 : 2;
+    int x, y;
+    if (r < 4 * TX) {                       // two rows above, two below
+        const int q = r / TX;
+        x = r - q * TX;
+        y = q < 2 ? q - 3 : TY + q - 1;     // -3, -2, TY + 1, TY + 2
+    } else if (r < 4 * TX + 4 * TY) {       // two columns left, two right
+        const int t = r - 4 * TX;
+        const int q = t / TY;
+        y = t - q * TY;
+        x = q < 2 ? q - 3 : TX + q - 1;     // -3, -2, TX + 1, TX + 2
+    } else {                                // corners: (-1,-1) (-2,-1) (-1,-2) and mirrored
+        const int t = r - 4 * TX - 4 * TY;  // 0 .. 11
+        const int corner = t / 3, k = t - 3 * corner;
+        const int dx = k == 1 ? 2 : 1, dy = k == 2 ? 2 : 1;
+        x = (corner & 1) ? TX - 1 + dx : -dx;
+        y = (corner & 2) ? TY - 1 + dy : -dy;
     }
+    xx = x + 3;
+    yy = y + 3;
 }
 
 // Stage A at one point: J - 1, error, weight and the nine weighted cofactors
 // W[3 r + a] = w * M_{a, r} (r: relation axis x, y, z).
 template <typename CT>
-__device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, int zz, int gx, int gy,
-                                               const float (&sm)[3][3][4], const float (&v0)[3], float T,
-                                               bool in_mask, bool count, double &Egrp, float &mx, CT (&W)[9],
-                                               bool zint)
+__device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, const float (&sm)[3][3][4],
+                                               const float (&v0)[3], float T, bool in_mask, bool count,
+                                               double &Egrp, float &mx, CT (&W)[9])
 {
-    // planes away from the z faces: the column's own coefficients (no
-    // wave-wide branch on boundary lanes); the four face planes: region map
-    const float dJ = zint ? jacobian_delta_coef<CT>(K, sm, v0)
-                          : jacobian_delta<CT>(P, gx, gy, zz, sm, v0);
+    // the stencil of every region as per-lane coefficients (no wave-wide
+    // branch on boundary lanes)
+    const float dJ = jacobian_delta_coef<CT>(K, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     const float ec = count ? e : 0.0f;
-    Egrp += (double)ec * (double)ec;
+    Egrp = fma_ct((double)ec, (double)ec, Egrp);
     mx = (count && in_mask) ? fmaxf(mx, fabsf(e)) : mx;
     const CT cw = (CT)penalised_weight(e, J, P);
     CT d2[3][3];
@@ -118,19 +123,19 @@ __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoe
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            d2[c][a] = (CT)(sm[c][a][2] - sm[c][a][1]) * (CT)P.c2[a] + (c == a ? (CT)1.0 : (CT)0.0);
+            d2[c][a] = fma_ct((CT)(sm[c][a][2] - sm[c][a][1]), (CT)P.c2[a], c == a ? (CT)1.0 : (CT)0.0);
     // relation axis x: minors from derivatives along y and z
-    W[0] = cw * (d2[1][1] * d2[2][2] - d2[1][2] * d2[2][1]);
-    W[1] = cw * (d2[0][1] * d2[2][2] - d2[0][2] * d2[2][1]);
-    W[2] = cw * (d2[0][1] * d2[1][2] - d2[0][2] * d2[1][1]);
+    W[0] = cw * fms2(d2[1][1], d2[2][2], d2[1][2], d2[2][1]);
+    W[1] = cw * fms2(d2[0][1], d2[2][2], d2[0][2], d2[2][1]);
+    W[2] = cw * fms2(d2[0][1], d2[1][2], d2[0][2], d2[1][1]);
     // relation axis y: minors from derivatives along x and z
-    W[3] = cw * (d2[1][0] * d2[2][2] - d2[1][2] * d2[2][0]);
-    W[4] = cw * (d2[0][0] * d2[2][2] - d2[0][2] * d2[2][0]);
-    W[5] = cw * (d2[0][0] * d2[1][2] - d2[0][2] * d2[1][0]);
+    W[3] = cw * fms2(d2[1][0], d2[2][2], d2[1][2], d2[2][0]);
+    W[4] = cw * fms2(d2[0][0], d2[2][2], d2[0][2], d2[2][0]);
+    W[5] = cw * fms2(d2[0][0], d2[1][2], d2[0][2], d2[1][0]);
     // relation axis z: minors from derivatives along x and y
-    W[6] = cw * (d2[1][0] * d2[2][1] - d2[1][1] * d2[2][0]);
-    W[7] = cw * (d2[0][0] * d2[2][1] - d2[0][1] * d2[2][0]);
-    W[8] = cw * (d2[0][0] * d2[1][1] - d2[0][1] * d2[1][0]);
+    W[6] = cw * fms2(d2[1][0], d2[2][1], d2[1][1], d2[2][0]);
+    W[7] = cw * fms2(d2[0][0], d2[2][1], d2[0][1], d2[2][0]);
+    W[8] = cw * fms2(d2[0][0], d2[1][1], d2[0][1], d2[1][0]);
 }
 
 // The sums are exact (fixed point, dt_device.hpp); their terms are the double
@@ -143,7 +148,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 {
     double Egrp = 0.0, Ggrp = 0.0;   // sums of the current group of planes
     using G = StreamGeom<TX, TY>;
-    constexpr int NT = G::NT, NOWN = G::NOWN, HX = G::HX, PL = G::PL, NRING = G::NRING;
+    constexpr int NT = G::NT, NOWN = G::NOWN, HX = G::HX, NRING = G::NRING;
     constexpr int NOTHER = G::NOTHER, SX = G::SX, SLOTS = G::SLOTS;
 
     const PassParams P = load_params(I);
@@ -174,14 +179,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const int tid = threadIdx.x;
     const bool own = (int)uni((unsigned)tid) < NOWN;
     const int rr = (tid - NOWN) % NRING;
-    int lx = 0, ly = 0, what = 0;   // what: 0 tile, 1 x-ring (W_{a,x}), 2 y-ring (W_{a,y})
+    int lx = 0, ly = 0;
     if (own) {
         lx = tid % TX; ly = tid / TX;
     } else {
-        if (rr < TY) { lx = -1; ly = rr; what = 1; }
-        else if (rr < 2 * TY) { lx = TX; ly = rr - TY; what = 1; }
-        else if (rr < 2 * TY + TX) { lx = rr - 2 * TY; ly = -1; what = 2; }
-        else { lx = rr - 2 * TY - TX; ly = TY; what = 2; }
+        if (rr < TY) { lx = -1; ly = rr; }
+        else if (rr < 2 * TY) { lx = TX; ly = rr - TY; }
+        else if (rr < 2 * TY + TX) { lx = rr - 2 * TY; ly = -1; }
+        else { lx = rr - 2 * TY - TX; ly = TY; }
     }
     const int gx = tc.x0 + lx, gy = tc.y0 + ly;
     if (own) { S.accE[tid] = 0; S.accG[tid] = 0; }
@@ -198,11 +203,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const bool pin = gx >= 0 && gx < nx && gy >= 0 && gy < ny;   // point in the volume
     const bool col_interior = own && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2;
     const unsigned pv = pin ? 4u * (unsigned)(gy * nx + gx) : 0u;
-    const ColCoef K = column_coefficients(P, gx, gy);
+    // Every LDS access of a step is the point's cell in one of the planes plus
+    // a constant: the W planes share the trial planes' pitch (their rows start
+    // at ly = -1), so one address register serves them all.  A ring point
+    // writes both relation axes into its own cells; the axis it does not serve
+    // is never read (stage B reads W_{a,x} at rows 0 .. TY-1 and W_{a,y} at
+    // columns 0 .. TX-1).
     const int pi = (ly + 3) * HX + (lx + 3);
-    // W cells of the primary point; a ring point's other relation axis: the sink
-    const int wxc = (own || what == 1) ? ly * (TX + 2) + lx + 1 : G::WXS - 1;
-    const int wyc = (own || what == 2) ? (ly + 1) * TX + lx : G::WYS - 1;
+    const int wi = pi - 2 * HX;
 
     // halo slots: a share of rings 2-3 and the ring-1 corners (slots past the
     // last point repeat points from the start)
@@ -225,19 +233,25 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // raw loads of one plane: the primary point's components (3) and the
     // halo slots; INTERP passes load base and candidate
     struct Raw { float a[SLOTS]; float b[INTERP ? SLOTS : 1]; };
-    auto fetch = [&](int gz, Raw &r) {
+    // (PART 1: the base buffer, PART 2: the candidate of an interpolated trial)
+    auto fetch_part = [&](auto part, int gz, Raw &r) {
+        constexpr int PART = decltype(part)::value;
         const int gzc = min(max(gz, 0), nz - 1);
         const unsigned so = 4u * (unsigned)gzc * plane;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r.a[c] = bload(r_src, pv, so + (unsigned)c * cn4);
-            if (INTERP) r.b[c] = bload(r_cnd, pv, so + (unsigned)c * cn4);
+            if (PART == 1) r.a[c] = bload(r_src, pv, so + (unsigned)c * cn4);
+            if (INTERP && PART == 2) r.b[c] = bload(r_cnd, pv, so + (unsigned)c * cn4);
         }
 #pragma unroll
         for (int j = 0; j < SX; ++j) {
-            r.a[3 + j] = bload(r_src, xv[j], so);
-            if (INTERP) r.b[3 + j] = bload(r_cnd, xv[j], so);
+            if (PART == 1) r.a[3 + j] = bload(r_src, xv[j], so);
+            if (INTERP && PART == 2) r.b[3 + j] = bload(r_cnd, xv[j], so);
         }
+    };
+    auto fetch = [&](int gz, Raw &r) {
+        fetch_part(std::integral_constant<int, 1>{}, gz, r);
+        fetch_part(std::integral_constant<int, 2>{}, gz, r);
     };
     // trial values of a plane (zero outside the volume)
     struct Val { float v[SLOTS]; };
@@ -246,7 +260,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
 #pragma unroll
         for (int k = 0; k < SLOTS; ++k) {
             float v = r.a[k];
-            if (INTERP) v = v + s * (r.b[k] - v);
+            if (INTERP) v = fma_ct(s, r.b[k] - v, v);
             const bool ok = zin && (k < 3 ? pin : ((xok >> (k - 3)) & 1u) != 0);
             o.v[k] = ok ? v : 0.0f;
         }
@@ -274,24 +288,26 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // only feeds stage B of planes below z0, which is skipped).
     const int zs = ((z1 - z0) & 1) ? z0 - 2 : z0 - 1;
     // LDS ring slot of plane z: (z - zs) mod 3
-    // prologue: planes zs and zs+1 committed (slots 0 and 1), plane zs+2 in flight
+    // prologue: planes zs and zs+1 committed (slots 0 and 1), plane zs+2 in
+    // flight.  The loads of all five planes are issued at once, so the
+    // prologue waits for one memory round trip, not one per plane.
     {
-        Raw r0, r1;
+        Raw r0, r1, r2, r3;
         fetch(zs - 2, r0);
         fetch(zs - 1, r1);
+        fetch(zs, r2);
+        fetch(zs + 1, r3);
         fetch_tm(zs, tma);
+        fetch(zs + 2, rn);
         Val v;
         resolve(zs - 2, r0, v, false);
         col[1][0] = v.v[0]; col[1][1] = v.v[1]; col[1][2] = v.v[2];
-        fetch(zs, r0);
         resolve(zs - 1, r1, v, false);
         col[2][0] = v.v[0]; col[2][1] = v.v[1]; col[2][2] = v.v[2];
-        fetch(zs + 1, r1);
-        resolve(zs, r0, v, false);
+        resolve(zs, r2, v, false);
         col[3][0] = v.v[0]; col[3][1] = v.v[1]; col[3][2] = v.v[2];
         commit(0, v);
-        fetch(zs + 2, rn);
-        resolve(zs + 1, r1, v, false);
+        resolve(zs + 1, r3, v, false);
         col[4][0] = v.v[0]; col[4][1] = v.v[1]; col[4][2] = v.v[2];
         commit(1, v);
         col[0][0] = col[0][1] = col[0][2] = 0.0f;
@@ -330,8 +346,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // step zz: plane zz+2 joins the window and the LDS ring, stage A on plane
     // zz (ring slot sa), stage B on plane zz-1.  The W planes alternate by the
     // step's place in its trip: PZ receives plane zz, 1 - PZ holds plane zz-1.
-    auto step = [&](auto fast, auto pz, int zz, int sa, TM &cur, TM &nxt) {
-        constexpr bool FAST = decltype(fast)::value;
+    // MODE 0: planes inside the z-chunk and off the z faces (no tests);
+    // MODE 1: stage A's plane off the z faces (2 <= zz <= nz - 3), chunk tests
+    // kept; MODE 2: any plane (the z faces take the region map).  Keeping the
+    // region map out of the first two keeps its registers (and their spills)
+    // out of the steps most chunks run.
+    auto step = [&](auto mode, auto pz, int zz, int sa, TM &cur, TM &nxt, const ColCoef &K) {
+        constexpr int MODE = decltype(mode)::value;
+        constexpr bool FAST = MODE == 0, OFFZ = MODE <= 1;
         constexpr int PZ = decltype(pz)::value;
         // The loads of plane zz+2 and of this plane's target / mask were issued
         // one step ago.  An empty asm that takes them here keeps the compiler
@@ -345,7 +367,7 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         asm volatile("" : "+v"(cur.t));
         {
             Val vz;
-            resolve(zz + 2, rn, vz, FAST);
+            resolve(zz + 2, rn, vz, OFFZ);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 col[0][c] = col[1][c]; col[1][c] = col[2][c]; col[2][c] = col[3][c]; col[3][c] = col[4][c];
@@ -354,13 +376,15 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             commit(sa == 0 ? 2 : sa - 1, vz);
         }
         store_pending(4u * (unsigned)max(zz - 2, 0) * plane);   // plane zz-2, stage B of the last step
-        fetch(zz + 3, rn);
+        // an interpolated trial's candidate plane is requested after stage A,
+        // where the most registers are live
+        fetch_part(std::integral_constant<int, 1>{}, zz + 3, rn);
         fetch_tm(zz + 1, nxt);
 
         // (A) plane zz at the primary point (computed at points outside the
         // volume too: nothing reads their W, their error is not counted)
         CT wz_p[3] = {0, 0, 0};
-        if (FAST || (zz >= 0 && zz < nz)) {
+        if (OFFZ || (zz >= 0 && zz < nz)) {
             const f3 *R = &S.xs[sa][pi];
             const f3 xm2 = R[-2], xm1 = R[-1], xp1 = R[1], xp2 = R[2];
             const f3 ym2 = R[-2 * HX], ym1 = R[-HX], yp1 = R[HX], yp2 = R[2 * HX];
@@ -373,25 +397,31 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             }
             const bool count = own && pin && (FAST || (zz >= z0 && zz < z1));
             CT W[9];
-            stream_stage_a<CT>(P, K, zz, gx, gy, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W,
-                               FAST || (zz >= 2 && zz <= nz - 3));
-            // (a ring point's other relation axis goes to the sink cell)
-            S.wx[PZ][wxc] = f3{(float)W[0], (float)W[1], (float)W[2]};
-            S.wy[PZ][wyc] = f3{(float)W[3], (float)W[4], (float)W[5]};
+            // (steps at the z faces rebuild the coefficients from the column:
+            // the column's own ones are not kept live through them)
+            const ColCoef KF = OFFZ ? K
+                                    : ((zz >= 2 && zz <= nz - 3) ? column_coefficients(P, gx, gy)
+                                                                 : face_coefficients(P, gx, gy, zz));
+            stream_stage_a<CT>(P, KF, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
+            // (a ring point also writes the relation axis it does not serve: never read)
+            S.w[PZ][0][wi] = f3{(float)W[0], (float)W[1], (float)W[2]};
+            S.w[PZ][1][wi] = f3{(float)W[3], (float)W[4], (float)W[5]};
 #pragma unroll
             for (int a = 0; a < 3; ++a) wz_p[a] = (CT)(float)W[6 + a];
         }
         if (FAST ? (zz & 3) == 3 : (zz >= z0 && zz < z1 && ((zz & 3) == 3 || zz == nz - 1)))
             flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
 
+        if (INTERP) fetch_part(std::integral_constant<int, 2>{}, zz + 3, rn);
+
         // (B) plane zb = zz-1 at tile columns: W_{a,x}, W_{a,y} of plane zb
         // (written last step), W_{a,z} of planes zb-1 and zb+1 (registers)
         const int zb = zz - 1;
         if (own && (FAST || (zb >= z0 && zb < z1))) {
             float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-            if (FAST || (zb >= 1 && zb <= nz - 2)) {
-                const f3 xl = S.wx[1 - PZ][wxc - 1], xr = S.wx[1 - PZ][wxc + 1];
-                const f3 yl = S.wy[1 - PZ][wyc - TX], yr = S.wy[1 - PZ][wyc + TX];
+            if (OFFZ || (zb >= 1 && zb <= nz - 2)) {
+                const f3 xl = S.w[1 - PZ][0][wi - 1], xr = S.w[1 - PZ][0][wi + 1];
+                const f3 yl = S.w[1 - PZ][1][wi - HX], yr = S.w[1 - PZ][1][wi + HX];
                 const CT dxx = (CT)xl[0] - (CT)xr[0];
                 const CT dyx = (CT)xl[1] - (CT)xr[1];
                 const CT dzx = (CT)xl[2] - (CT)xr[2];
@@ -399,16 +429,18 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
                 const CT dyy = (CT)yl[1] - (CT)yr[1];
                 const CT dzy = (CT)yl[2] - (CT)yr[2];
                 // the gradient is zero on the outer face (and unused outside the volume)
-                f0 = col_interior ? (float)(cix * dxx - ciy * dxy + ciz * (wz_m[0] - wz_p[0])) : 0.0f;
-                f1 = col_interior ? (float)(-cix * dyx + ciy * dyy - ciz * (wz_m[1] - wz_p[1])) : 0.0f;
-                f2 = col_interior ? (float)(cix * dzx - ciy * dzy + ciz * (wz_m[2] - wz_p[2])) : 0.0f;
-                Ggrp += (double)f0 * f0 + (double)f1 * f1 + (double)f2 * f2;
+                f0 = col_interior ? (float)fma_ct(cix, dxx, fma_ct(-ciy, dxy, ciz * (wz_m[0] - wz_p[0]))) : 0.0f;
+                f1 = col_interior ? (float)fma_ct(-cix, dyx, fma_ct(ciy, dyy, -ciz * (wz_m[1] - wz_p[1]))) : 0.0f;
+                f2 = col_interior ? (float)fma_ct(cix, dzx, fma_ct(-ciy, dzy, ciz * (wz_m[2] - wz_p[2]))) : 0.0f;
+                Ggrp += fma_ct((double)f0, (double)f0, fma_ct((double)f1, (double)f1, (double)f2 * f2));
             }
             // trial value of this voxel on plane zb: window slot 1 (planes zz-2 .. zz+2);
             // stored at the start of the next step
             const float ux = col[1][0], uy = col[1][1], uz = col[1][2];
             pend.on = pin;
-            pend.n[0] = ux - eta_next * f0; pend.n[1] = uy - eta_next * f1; pend.n[2] = uz - eta_next * f2;
+            pend.n[0] = fma_ct(-eta_next, f0, ux);
+            pend.n[1] = fma_ct(-eta_next, f1, uy);
+            pend.n[2] = fma_ct(-eta_next, f2, uz);
         }
         if (FAST ? (zb & 3) == 3 : (zb >= z0 && zb < z1 && ((zb & 3) == 3 || zb == nz - 1)))
             flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
@@ -421,26 +453,32 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // the trip is not unrolled further (more steps in flight would raise the
     // live registers)
     int sa = 0;
-    // trips (zz, zz + 1) whose planes are inside the chunk and the volume and
-    // off the z faces take the step with those tests folded away
-    const int flo = max(z0 + 1, 2), fhi = min(z1 - 1, nz - 3);
-    const int t_lo = (flo - zs + 1) >> 1;
-    const int t_hi = fhi - 1 - zs >= 0 ? (fhi - 1 - zs) >> 1 : -1;
-    int t = 0;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    auto trip = [&](auto mode, int zz, const ColCoef &K) {
+        step(mode, P0{}, zz, sa, tma, tmb, K);
+        sa = sa == 2 ? 0 : sa + 1;
+        step(mode, P1{}, zz + 1, sa, tmb, tma, K);
+        sa = sa == 2 ? 0 : sa + 1;
+    };
+    // Three phases: trips touching the z faces below, trips off the z faces
+    // (those inside the chunk with their tests folded away, the others with
+    // the chunk tests only), trips touching the z faces above.  The column's
+    // coefficients live through the middle phase alone.
+    int zz = zs;
 #pragma unroll 1
-    for (int zz = zs; zz < z1; zz += 2, ++t) {
-        if (t >= t_lo && t <= t_hi) {
-            step(std::true_type{}, std::integral_constant<int, 0>{}, zz, sa, tma, tmb);
-            sa = sa == 2 ? 0 : sa + 1;
-            step(std::true_type{}, std::integral_constant<int, 1>{}, zz + 1, sa, tmb, tma);
-            sa = sa == 2 ? 0 : sa + 1;
-        } else {
-            step(std::false_type{}, std::integral_constant<int, 0>{}, zz, sa, tma, tmb);
-            sa = sa == 2 ? 0 : sa + 1;
-            step(std::false_type{}, std::integral_constant<int, 1>{}, zz + 1, sa, tmb, tma);
-            sa = sa == 2 ? 0 : sa + 1;
+    for (; zz < z1 && zz < 2; zz += 2) trip(std::integral_constant<int, 2>{}, zz, ColCoef{});
+    {
+        const ColCoef K = column_coefficients(P, gx, gy);
+        const int flo = max(z0 + 1, 2), fhi = min(z1 - 1, nz - 3);
+#pragma unroll 1
+        for (; zz < z1 && zz + 1 <= nz - 3; zz += 2) {
+            if (zz >= flo && zz + 1 <= fhi) trip(std::integral_constant<int, 0>{}, zz, K);
+            else trip(std::integral_constant<int, 1>{}, zz, K);
         }
     }
+#pragma unroll 1
+    for (; zz < z1; zz += 2) trip(std::integral_constant<int, 2>{}, zz, ColCoef{});
     // the last step's outputs: shift the window as a next step would
 #pragma unroll
     for (int c = 0; c < 3; ++c) col[0][c] = col[1][c];
@@ -461,7 +499,7 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, in
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl, S.acc_w);
+    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl);
     __syncthreads();
     const Ctrl *C = &S.ctrl;
     if (!uni(C->active)) {

@@ -54,7 +54,6 @@ struct GreedySmem {
     float us[3][3][G::RP];         // trial planes, ring of three
     SumsWave red[G::NT / 64];
     Ctrl ctrl;                     // this pass's control state (pass_begin)
-    unsigned long long acc_w[BEGIN_ACC_WORDS];   // the instance's accumulators (pass_begin)
 };
 
 // Region coordinates (rx, ry) of the r-th point of the region ring (outside
@@ -337,7 +336,7 @@ k_greedy_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, 
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    if (threadIdx.x < 64) pass_begin<DT_GREEDY>(I, parity, leader_tile(tl), &S.ctrl, S.acc_w);
+    if (threadIdx.x < 64) pass_begin<DT_GREEDY>(I, parity, leader_tile(tl), &S.ctrl);
     __syncthreads();
     const Ctrl *C = &S.ctrl;
     if (!uni(C->active)) {

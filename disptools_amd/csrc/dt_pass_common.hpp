@@ -108,9 +108,27 @@ __device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, int 
     return K;
 }
 
-// Fused multiply-add in the arithmetic type (__builtin_fma is the double one).
-__device__ __forceinline__ float fma_ct(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fma_ct(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// Coefficients of column (gx, gy) on a plane zz at the z faces (zz < 2 or
+// zz > nz - 3).  There the region map gives every axis the outer-face rule
+// (src/jacobian.c:112-241): one-sided along the axes where the voxel lies on
+// the outer face, 2nd-order central along the others (on planes 1 and
+// nz - 2 off the x / y faces that is the penultimate layer's 2nd-order
+// stencil, the same thing).  The values are column_coefficients' for those
+// kinds.
+__device__ __forceinline__ ColCoef face_coefficients(const PassParams &P, int gx, int gy, int zz)
+{
+    const int g[3] = {gx, gy, zz}, n[3] = {P.nx, P.ny, P.nz};
+    ColCoef F;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool lo = g[a] == 0, hi = g[a] == n[a] - 1;
+        const float inv = P.inv[a];
+        F.A[a] = (lo || hi) ? 2.0f * inv : P.c2[a];
+        F.B[a] = (lo || hi) ? -0.5f * inv : 0.0f;
+        F.G[a] = lo ? -1.5f * inv : (hi ? 1.5f * inv : 0.0f);
+    }
+    return F;
+}
 
 // J - 1 from the column's coefficients (planes 2 .. nz - 3).  A lane with the
 // 4th-order stencil rounds as jacobian_delta's interior form (G c = 0 adds
@@ -135,7 +153,7 @@ __device__ __forceinline__ float jacobian_delta_coef(const ColCoef &K, const flo
 // w = e + zeta (J - delta) where J < delta.
 __device__ __forceinline__ float penalised_weight(float e, float J, const PassParams &P)
 {
-    return J < P.delta ? e + P.zeta * (J - P.delta) : e;
+    return J < P.delta ? fma_ct(P.zeta, J - P.delta, e) : e;
 }
 
 }  // namespace dt

@@ -120,18 +120,33 @@ __host__ __device__ __forceinline__ CT det_plus_identity(const CT d[3][3])
          + a13 * (a21 * a32 - a22 * a31);
 }
 
+// Fused multiply-add in the arithmetic type (__builtin_fma is the double one).
+__device__ __forceinline__ float fma_ct(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_ct(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// a * b - c * d as one fused multiply-add over the rounded product c * d.
+template <typename CT>
+__device__ __forceinline__ CT fms2(CT a, CT b, CT c, CT d)
+{
+#pragma clang fp contract(off)
+    return fma_ct(a, b, -(c * d));
+}
+
 // det(I + D) - 1 = tr D + (sum of the principal 2x2 minors of D) + det D,
 // summed from the smallest terms up.  For |D| << 1 this keeps J - 1 to a
 // relative accuracy of the compute type instead of an absolute error of half
-// an ulp of 1.0.
+// an ulp of 1.0.  Every fused multiply-add is written out and nothing else
+// is contracted, so every compilation of a pass rounds it the same way.
 template <typename CT>
 __device__ __forceinline__ CT det_plus_identity_minus_one(const CT d[3][3])
 {
-    const CT m01 = d[0][0] * d[1][1] - d[0][1] * d[1][0];
-    const CT m02 = d[0][0] * d[2][2] - d[0][2] * d[2][0];
-    const CT m12 = d[1][1] * d[2][2] - d[1][2] * d[2][1];
-    const CT det = d[0][0] * m12 - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
-                   d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+#pragma clang fp contract(off)
+    const CT m01 = fms2(d[0][0], d[1][1], d[0][1], d[1][0]);
+    const CT m02 = fms2(d[0][0], d[2][2], d[0][2], d[2][0]);
+    const CT m12 = fms2(d[1][1], d[2][2], d[1][2], d[2][1]);
+    const CT c1 = fms2(d[1][0], d[2][2], d[1][2], d[2][0]);
+    const CT c2 = fms2(d[1][0], d[2][1], d[1][1], d[2][0]);
+    const CT det = fma_ct(d[0][0], m12, fma_ct(-d[0][1], c1, d[0][2] * c2));
     return ((det + (m01 + m02 + m12)) + (d[0][0] + d[1][1] + d[2][2]));
 }
 
