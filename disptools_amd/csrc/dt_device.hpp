@@ -524,7 +524,8 @@ static __device__ __forceinline__ void greedy_controller(const CtlEnv &V, Ctrl *
 // The start of a pass
 // ---------------------------------------------------------------------------
 
-// Run by wave 0 of every block before the block's work: reads the control
+// The controller step (pass_begin_load, then pass_begin_run), run by wave 0
+// of every block before the block's work: reads the control
 // block of this pass's parity and, when it records an evaluated trial,
 // advances the controller (ALG: DT_GRADIENT or DT_GREEDY) by the sums of that
 // pass, then leaves the state in `c` (the block's LDS copy).  Every block
@@ -541,8 +542,20 @@ static __device__ __forceinline__ void greedy_controller(const CtlEnv &V, Ctrl *
 // block is gathered into wave-uniform registers (one word per lane, read
 // across lanes) and the controller runs there: a controller on an LDS copy
 // waits for one LDS round trip per field it touches.
-template <int ALG>
-__device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
+// What wave 0 loads for the controller step: its lane's control or CtlEnv
+// word and accumulator record, and the instance's control copies and
+// accumulators (kept in registers for the leader's stores: the compiler
+// would otherwise reload them from the instance record there, a round trip
+// of its own).
+struct BeginLoads {
+    unsigned long long cw, aw[8], a8;
+    Ctrl *ctrl;
+    Acc *acc;
+};
+
+// The loads of the controller step (pass_begin_run consumes them); a kernel
+// may issue other loads in between, whose round trip then overlaps this one.
+__device__ __forceinline__ BeginLoads pass_begin_load(const Inst &I, int parity)
 {
     constexpr int W = (int)(sizeof(Ctrl) / 8);
     constexpr int PW = (int)(sizeof(dt_params) / 8), VW = (int)(sizeof(CtlEnv) / 8);
@@ -551,14 +564,12 @@ __device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
     static_assert(sizeof(dt_params) % 8 == 0 && offsetof(CtlEnv, trace) == sizeof(dt_params), "CtlEnv layout");
     static_assert(offsetof(Inst, final_ctrl) - offsetof(Inst, trace) == 24, "Inst::trace .. final_ctrl");
     const int lane = threadIdx.x & 63;
-    // the instance's control copies and accumulators, kept in registers for
-    // the leader's stores (the compiler would otherwise reload them from the
-    // instance record there, a round trip of its own)
-    Ctrl *ctrl = I.ctrl;
-    Acc *acc = I.acc;
-    asm volatile("" : "+s"(ctrl), "+s"(acc));
-    const unsigned long long *src = (const unsigned long long *)(ctrl + parity);
-    const unsigned long long *rec = (const unsigned long long *)(acc + (lane < NREC ? lane : 0));
+    BeginLoads L;
+    L.ctrl = I.ctrl;
+    L.acc = I.acc;
+    asm volatile("" : "+s"(L.ctrl), "+s"(L.acc));
+    const unsigned long long *src = (const unsigned long long *)(L.ctrl + parity);
+    const unsigned long long *rec = (const unsigned long long *)(L.acc + (lane < NREC ? lane : 0));
     // lane k < W: word k of the control block; W <= k < W + VW: word k - W
     // of the CtlEnv (the parameters, then Inst::trace .. Inst::final_ctrl)
     // (every lane loads from a valid address, so the loads need no branches)
@@ -566,11 +577,27 @@ __device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
                                  : lane < W + PW ? (const unsigned long long *)&I.p + (lane - W)
                                  : lane < W + VW ? (const unsigned long long *)&I.trace + (lane - W - PW)
                                                  : src;
-    const unsigned long long cw = *wp;
+    L.cw = *wp;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) L.aw[k] = rec[k];
+    L.a8 = rec[8];
+    return L;
+}
+
+template <int ALG>
+__device__ void pass_begin_run(const Inst &I, int parity, bool leader, Ctrl *c, const BeginLoads &L)
+{
+    constexpr int W = (int)(sizeof(Ctrl) / 8);
+    constexpr int VW = (int)(sizeof(CtlEnv) / 8);
+    constexpr int NREC = ACC_SLOTS * ACC_COPIES;
+    const int lane = threadIdx.x & 63;
+    Ctrl *ctrl = L.ctrl;
+    Acc *acc = L.acc;
+    const unsigned long long cw = L.cw;
     unsigned long long aw[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) aw[k] = rec[k];
-    const unsigned long long a8 = rec[8];
+    for (int k = 0; k < 8; ++k) aw[k] = L.aw[k];
+    const unsigned long long a8 = L.a8;
     const bool has_rec = lane < NREC;
 #pragma unroll
     for (int k = 0; k < 8; ++k) aw[k] = has_rec ? aw[k] : 0ull;
@@ -616,6 +643,13 @@ __device__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
         }
         if (lane == 0) publish_status(V, &s);
     }
+}
+
+template <int ALG>
+__device__ __forceinline__ void pass_begin(const Inst &I, int parity, bool leader, Ctrl *c)
+{
+    const BeginLoads L = pass_begin_load(I, parity);
+    pass_begin_run<ALG>(I, parity, leader, c, L);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,7 +49,14 @@ template <int TX, int TY>
 struct StreamGeom {
     static constexpr int NOWN = TX * TY;                          // tile threads
     static constexpr int NRING = 2 * TX + 2 * TY;                 // ring-1 points without corners
-    static constexpr int NT = NOWN + (NRING + 63) / 64 * 64;      // block size
+    static constexpr int NT = NOWN + (NRING + 63) / 64 * 64;      // threads with a point
+    // The controller step gets a wave of its own, launched after the point
+    // waves, where one more wave does not raise the waves per SIMD (and so the
+    // register budget): its registers then never meet the plane loop's, and
+    // the point waves issue their first loads while it runs.
+    static constexpr int NW = NT / 64;
+    static constexpr bool CW = (NW + 4) / 4 == (NW + 3) / 4;
+    static constexpr int NL = NT + (CW ? 64 : 0);                 // threads launched
     static constexpr int HX = TX + 6, HY = TY + 6, PL = HX * HY;
     // the rest of the stencils' reach: rings 2-3 along the tile's sides and
     // 12 cells at the corners (the rings' outer corner cells are never read)
@@ -104,14 +111,14 @@ __device__ __forceinline__ void other_point(int r, int &xx, int &yy)
 
 // Stage A at one point: J - 1, error, weight and the nine weighted cofactors
 // W[3 r + a] = w * M_{a, r} (r: relation axis x, y, z).
-template <typename CT>
+template <typename CT, bool ZFACE>
 __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoef &K, const float (&sm)[3][3][4],
                                                const float (&v0)[3], float T, bool in_mask, bool count,
                                                double &Egrp, float &mx, CT (&W)[9])
 {
     // the stencil of every region as per-lane coefficients (no wave-wide
     // branch on boundary lanes)
-    const float dJ = jacobian_delta_coef<CT>(K, sm, v0);
+    const float dJ = jacobian_delta_coef<CT, ZFACE>(K, sm, v0);
     const float J = 1.0f + dJ;
     const float e = voxel_error_delta(dJ, T - 1.0f, in_mask, P.tol);
     const float ec = count ? e : 0.0f;
@@ -142,10 +149,19 @@ __device__ __forceinline__ void stream_stage_a(const PassParams &P, const ColCoe
 // sums of e^2 and |g|^2 over groups of four planes of one column, the groups
 // starting at z = 0 mod 4 (z-chunks start on group boundaries), so the sums
 // do not depend on the tiling.
-template <typename CT, int TX, int TY, bool INTERP>
-__device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const Inst &I, const Ctrl *C,
-                                                 const TileCoord tc, unsigned &flags, float &mx)
+//
+// A direct trial's body (INTERP false) starts before the controller step has
+// chosen the trial: it issues the prologue's loads from buffer `guess`, then
+// calls begin() (the controller step and a block barrier) and reads the
+// state.  It returns false, having done nothing more, when the pass turns out
+// inactive or interpolated; it reloads the prologue when the trial is another
+// buffer.  An interpolated trial's body runs once the state is known (begin
+// does nothing there).
+template <typename CT, int TX, int TY, bool INTERP, typename Begin>
+__device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const Inst &I, const TileCoord tc, int guess,
+                                                 Begin &&begin, unsigned &flags, float &mx)
 {
+    const Ctrl *C = &S.ctrl;
     double Egrp = 0.0, Ggrp = 0.0;   // sums of the current group of planes
     using G = StreamGeom<TX, TY>;
     constexpr int NT = G::NT, NOWN = G::NOWN, HX = G::HX, NRING = G::NRING;
@@ -156,18 +172,16 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     const unsigned n = uni((unsigned)I.n);
     const unsigned plane = (unsigned)nx * (unsigned)ny;
     const unsigned fbytes = 12u * n;
-    const int b_src = uni(C->src), b_cnd = uni(INTERP ? C->cand : C->src);
-    const int b_next = uni(C->out_next), b_trial = uni(C->out_trial);
-    const bool has_next = b_next >= 0, has_trial = INTERP && b_trial >= 0;
-    float *p_src = uni(I.buf[b_src]);
-    const __amdgpu_buffer_rsrc_t r_src = make_rsrc(p_src, fbytes);
-    const __amdgpu_buffer_rsrc_t r_cnd = make_rsrc(uni(I.buf[b_cnd]), fbytes);
+    int b_src = INTERP ? uni(C->src) : guess;
+    __amdgpu_buffer_rsrc_t r_src = make_rsrc(uni(I.buf[b_src]), fbytes);
+    const __amdgpu_buffer_rsrc_t r_cnd = INTERP ? make_rsrc(uni(I.buf[uni(C->cand)]), fbytes) : r_src;
     const __amdgpu_buffer_rsrc_t r_tgt = make_rsrc(uni(I.target), 4u * n);
     const __amdgpu_buffer_rsrc_t r_msk = make_rsrc(uni(I.mask), n);
-    const __amdgpu_buffer_rsrc_t r_next = make_rsrc(has_next ? uni(I.buf[b_next]) : p_src, fbytes);
-    const __amdgpu_buffer_rsrc_t r_trial = make_rsrc(has_trial ? uni(I.buf[b_trial]) : p_src, fbytes);
-    const float s = uni(C->s);
-    const float eta_next = uni(C->eta_next);
+    const float s = INTERP ? uni(C->s) : 0.0f;
+    // outputs, set once the state is known
+    __amdgpu_buffer_rsrc_t r_next = r_src, r_trial = r_src;
+    bool has_next = false, has_trial = false;
+    float eta_next = 0.0f;
     const int z0 = tc.z0, z1 = tc.z1;
     const unsigned cn4 = 4u * n;   // bytes between components
 
@@ -290,7 +304,8 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // LDS ring slot of plane z: (z - zs) mod 3
     // prologue: planes zs and zs+1 committed (slots 0 and 1), plane zs+2 in
     // flight.  The loads of all five planes are issued at once, so the
-    // prologue waits for one memory round trip, not one per plane.
+    // prologue waits for one memory round trip, not one per plane; for a
+    // direct trial that round trip overlaps the controller step.
     {
         Raw r0, r1, r2, r3;
         fetch(zs - 2, r0);
@@ -299,6 +314,28 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
         fetch(zs + 1, r3);
         fetch_tm(zs, tma);
         fetch(zs + 2, rn);
+        begin();
+        if (!INTERP) {
+            if (!uni(C->active) || uni(C->mode) != TRIAL_DIRECT) return false;
+            const int b = uni(C->src);
+            if (b != b_src) {
+                b_src = b;
+                r_src = make_rsrc(uni(I.buf[b]), fbytes);
+                fetch(zs - 2, r0);
+                fetch(zs - 1, r1);
+                fetch(zs, r2);
+                fetch(zs + 1, r3);
+                fetch(zs + 2, rn);
+            }
+        }
+        {
+            const int b_next = uni(C->out_next), b_trial = uni(C->out_trial);
+            has_next = b_next >= 0;
+            has_trial = INTERP && b_trial >= 0;
+            if (has_next) r_next = make_rsrc(uni(I.buf[b_next]), fbytes);
+            if (has_trial) r_trial = make_rsrc(uni(I.buf[b_trial]), fbytes);
+            eta_next = uni(C->eta_next);
+        }
         Val v;
         resolve(zs - 2, r0, v, false);
         col[1][0] = v.v[0]; col[1][1] = v.v[1]; col[1][2] = v.v[2];
@@ -399,10 +436,14 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
             CT W[9];
             // (steps at the z faces rebuild the coefficients from the column:
             // the column's own ones are not kept live through them)
-            const ColCoef KF = OFFZ ? K
-                                    : ((zz >= 2 && zz <= nz - 3) ? column_coefficients(P, gx, gy)
-                                                                 : face_coefficients(P, gx, gy, zz));
-            stream_stage_a<CT>(P, KF, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
+            if (OFFZ)
+                stream_stage_a<CT, false>(P, K, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
+            else if (zz >= 2 && zz <= nz - 3)
+                stream_stage_a<CT, false>(P, column_coefficients(P, gx, gy), sm, col[2], cur.t, cur.m != 0,
+                                          count, Egrp, mx, W);
+            else
+                stream_stage_a<CT, true>(P, face_coefficients(P, gx, gy, zz), sm, col[2], cur.t, cur.m != 0,
+                                         count, Egrp, mx, W);
             // (a ring point also writes the relation axis it does not serve: never read)
             S.w[PZ][0][wi] = f3{(float)W[0], (float)W[1], (float)W[2]};
             S.w[PZ][1][wi] = f3{(float)W[3], (float)W[4], (float)W[5]};
@@ -487,34 +528,55 @@ __device__ __forceinline__ void stream_pass_body(StreamSmem<TX, TY> &S, const In
     // its remainder
     flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
     flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
+    return true;
 }
 
 template <typename CT, int TX, int TY, int MINW>
-__global__ void __launch_bounds__((StreamGeom<TX, TY>::NT), MINW)
+__global__ void __launch_bounds__((StreamGeom<TX, TY>::NL), MINW)
 k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, int parity)
 {
     constexpr int NT = StreamGeom<TX, TY>::NT;
     __shared__ StreamSmem<TX, TY> S;
 
+    using G = StreamGeom<TX, TY>;
     const Tile tl = tiles[blockIdx.x];
     if (tl.inst < 0) return;
     const Inst &I = insts[tl.inst];
-    if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl);
-    __syncthreads();
-    const Ctrl *C = &S.ctrl;
-    if (!uni(C->active)) {
-        if (uni(C->copy_now)) copy_result_tile<TX, TY, NT>(I, C, decode_tile<TX, TY>(I, tl));
-        return;
-    }
     TileCoord tc = decode_tile<TX, TY>(I, tl);
     tc.x0 = uni(tc.x0); tc.y0 = uni(tc.y0); tc.z0 = uni(tc.z0); tc.z1 = uni(tc.z1);
-
     unsigned flags = 0;
     float mx = 0.0f;
-    if (uni(C->mode) == TRIAL_INTERP)
-        stream_pass_body<CT, TX, TY, true>(S, I, C, tc, flags, mx);
-    else
-        stream_pass_body<CT, TX, TY, false>(S, I, C, tc, flags, mx);
+    bool done = false;
+    const Ctrl *C = &S.ctrl;
+    if (G::CW) {
+        if (threadIdx.x >= NT) {   // the controller wave: its step, the barrier that publishes it, done
+            pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl);
+            __syncthreads();
+            return;
+        }
+        // This pass's control copy before the controller step (no block
+        // writes it during the pass).  The trial a pass evaluates is most
+        // often the candidate the previous pass wrote, the next trial after an
+        // accepted one (the first pass: the initial field).
+        const Ctrl *cg = I.ctrl + parity;
+        int guess = uni(cg->evaluated) ? uni(cg->out_next) : uni(cg->src);
+        guess = (guess >= 0 && guess < NBUF) ? guess : 0;
+        if (uni(cg->active)) done = stream_pass_body<CT, TX, TY, false>(S, I, tc, guess, [] { __syncthreads(); },
+                                                                         flags, mx);
+        else __syncthreads();
+    } else {
+        if (threadIdx.x < 64) pass_begin<DT_GRADIENT>(I, parity, leader_tile(tl), &S.ctrl);
+        __syncthreads();
+        if (uni(C->active) && uni(C->mode) == TRIAL_DIRECT)
+            done = stream_pass_body<CT, TX, TY, false>(S, I, tc, uni(C->src), [] {}, flags, mx);
+    }
+    if (!done) {
+        if (!uni(C->active)) {
+            if (uni(C->copy_now)) copy_result_tile<TX, TY, NT>(I, C, tc);
+            return;
+        }
+        stream_pass_body<CT, TX, TY, true>(S, I, tc, 0, [] {}, flags, mx);
+    }
 
     add_pass_sums_lds<NT, StreamGeom<TX, TY>::NOWN>(I, uni(C->acc_cur), S.accE, S.accG, mx, flags, S.red);
 }
@@ -532,18 +594,18 @@ k_grad_stream(const Inst *__restrict__ insts, const Tile *__restrict__ tiles, in
 hipError_t DT_PASTE(stream_launch_, DT_STREAM_NAME)(const Inst *insts, const Tile *tiles, int nblocks, int parity,
                                                      hipStream_t st)
 {
-    constexpr int NT = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NT;
+    constexpr int NL = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NL;
     hipLaunchKernelGGL((k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>), dim3(nblocks),
-                       dim3(NT), 0, st, insts, tiles, parity);
+                       dim3(NL), 0, st, insts, tiles, parity);
     return hipGetLastError();
 }
 
 int DT_PASTE(stream_occupancy_, DT_STREAM_NAME)()
 {
-    constexpr int NT = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NT;
+    constexpr int NL = StreamGeom<DT_STREAM_TX, DT_STREAM_TY>::NL;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &blocks, k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>, NT, 0) != hipSuccess)
+            &blocks, k_grad_stream<DT_STREAM_CT, DT_STREAM_TX, DT_STREAM_TY, DT_STREAM_MINW>, NL, 0) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
 }

@@ -130,10 +130,11 @@ __device__ __forceinline__ ColCoef face_coefficients(const PassParams &P, int gx
     return F;
 }
 
-// J - 1 from the column's coefficients (planes 2 .. nz - 3).  A lane with the
-// 4th-order stencil rounds as jacobian_delta's interior form (G c = 0 adds
-// an exact zero).
-template <typename CT>
+// J - 1 from per-lane stencil coefficients.  A lane with the 4th-order
+// stencil rounds as jacobian_delta's interior form (G c = 0 adds an exact
+// zero).  ZFACE: the plane is at a z face, where the z stencil may use the
+// centre sample; elsewhere it never does (G[2] = 0) and that term is left out.
+template <typename CT, bool ZFACE = true>
 __device__ __forceinline__ float jacobian_delta_coef(const ColCoef &K, const float (&sm)[3][3][4],
                                                      const float (&v0)[3])
 {
@@ -144,7 +145,8 @@ __device__ __forceinline__ float jacobian_delta_coef(const ColCoef &K, const flo
         for (int a = 0; a < 3; ++a) {
             const CT D1 = (CT)(sm[c][a][2] - sm[c][a][1]);
             const CT D2 = (CT)(sm[c][a][3] - sm[c][a][0]);
-            d4[c][a] = fma_ct(D1, (CT)K.A[a], fma_ct(D2, (CT)K.B[a], (CT)K.G[a] * (CT)v0[c]));
+            const CT g = (a < 2 || ZFACE) ? fma_ct(D2, (CT)K.B[a], (CT)K.G[a] * (CT)v0[c]) : D2 * (CT)K.B[a];
+            d4[c][a] = fma_ct(D1, (CT)K.A[a], g);
         }
     return (float)det_plus_identity_minus_one<CT>(d4);
 }
