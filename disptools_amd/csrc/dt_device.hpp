@@ -153,12 +153,20 @@ struct SumsWave {
     unsigned flags;
 };
 
-// One thread adds a block's reduced sums into the block's copy of
-// accumulator slot `slot` (relaxed, agent scope: the pass that reads them
-// starts after this kernel has ended).
-__device__ __forceinline__ void commit_block_sums(const Inst &I, int slot, u128 e, u128 g, float mx, unsigned flags)
+// The block's copy of accumulator slot `slot`, read from the instance record
+// at once and kept in registers (a use at the end of the pass would reload it
+// there, a round trip of its own).
+__device__ __forceinline__ DT_GLOBAL Acc *block_acc(const Inst &I, int slot)
 {
-    Acc *a = I.acc + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
+    DT_GLOBAL Acc *a = gptr(I.acc) + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
+    asm volatile("" : "+s"(a));
+    return a;
+}
+
+// One thread adds a block's reduced sums into its accumulator copy (relaxed,
+// agent scope: the pass that reads them starts after this kernel has ended).
+__device__ __forceinline__ void commit_block_sums(DT_GLOBAL Acc *a, u128 e, u128 g, float mx, unsigned flags)
+{
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const unsigned long long le = (unsigned long long)(unsigned)(e >> (32 * k));
@@ -178,6 +186,7 @@ __device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx,
 {
     constexpr int NW = NT / 64;
     static_assert(NW <= 64, "one wave's results per lane");
+    DT_GLOBAL Acc *acc = block_acc(I, slot);
     e = wave_sum_u128(e);
     g = wave_sum_u128(g);
     const unsigned m = wave_max_u32(__float_as_uint(mx));
@@ -199,7 +208,7 @@ __device__ void add_pass_sums(const Inst &I, int slot, u128 e, u128 g, float mx,
     sg = wave_sum_u128(sg);
     sm = wave_max_u32(sm);
     sf = wave_or_u32(sf);
-    if (lane == 0) commit_block_sums(I, slot, se, sg, __uint_as_float(sm), sf);
+    if (lane == 0) commit_block_sums(acc, se, sg, __uint_as_float(sm), sf);
 }
 
 // Adds a block's sums into its accumulator copy when the exact sums sit in
@@ -223,6 +232,7 @@ __device__ void add_pass_sums_lds(const Inst &I, int slot, const u128 *accE, con
     }
     __syncthreads();
     if (threadIdx.x >= 64) return;
+    DT_GLOBAL Acc *acc = block_acc(I, slot);
     u128 e = 0, g = 0;
 #pragma unroll 4
     for (int i = lane; i < NOWN; i += 64) {
@@ -235,7 +245,7 @@ __device__ void add_pass_sums_lds(const Inst &I, int slot, const u128 *accE, con
     g = wave_sum_u128(g);
     sm = wave_max_u32(sm);
     sf = wave_or_u32(sf);
-    if (lane == 0) commit_block_sums(I, slot, e, g, __uint_as_float(sm), sf);
+    if (lane == 0) commit_block_sums(acc, e, g, __uint_as_float(sm), sf);
 }
 
 // 128-bit fixed-point value (quantum 2^-64) as a double; deterministic.

@@ -392,6 +392,13 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
         constexpr int MODE = decltype(mode)::value;
         constexpr bool FAST = MODE == 0, OFFZ = MODE <= 1;
         constexpr int PZ = decltype(pz)::value;
+        // Steps at the z faces rebuild their stencil coefficients from the
+        // column (the column's own ones are not kept live through them), here,
+        // before the step's loads are issued: a register the compiler reloads
+        // from scratch for this then waits only for loads already due.
+        const bool zint = OFFZ || (zz >= 2 && zz <= nz - 3);
+        ColCoef KF;
+        if (!OFFZ) KF = zint ? column_coefficients(P, gx, gy) : face_coefficients(P, gx, gy, zz);
         // The loads of plane zz+2 and of this plane's target / mask were issued
         // one step ago.  An empty asm that takes them here keeps the compiler
         // from hoisting their first use above the barrier that ended the last
@@ -434,16 +441,8 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
             }
             const bool count = own && pin && (FAST || (zz >= z0 && zz < z1));
             CT W[9];
-            // (steps at the z faces rebuild the coefficients from the column:
-            // the column's own ones are not kept live through them)
-            if (OFFZ)
-                stream_stage_a<CT, false>(P, K, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
-            else if (zz >= 2 && zz <= nz - 3)
-                stream_stage_a<CT, false>(P, column_coefficients(P, gx, gy), sm, col[2], cur.t, cur.m != 0,
-                                          count, Egrp, mx, W);
-            else
-                stream_stage_a<CT, true>(P, face_coefficients(P, gx, gy, zz), sm, col[2], cur.t, cur.m != 0,
-                                         count, Egrp, mx, W);
+            if (zint) stream_stage_a<CT, false>(P, OFFZ ? K : KF, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
+            else stream_stage_a<CT, true>(P, KF, sm, col[2], cur.t, cur.m != 0, count, Egrp, mx, W);
             // (a ring point also writes the relation axis it does not serve: never read)
             S.w[PZ][0][wi] = f3{(float)W[0], (float)W[1], (float)W[2]};
             S.w[PZ][1][wi] = f3{(float)W[3], (float)W[4], (float)W[5]};
@@ -525,9 +524,9 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
     for (int c = 0; c < 3; ++c) col[0][c] = col[1][c];
     store_pending(4u * (unsigned)max(z1 - 1, 0) * plane);
     // a chunk that does not end on a group boundary (not planned so) keeps
-    // its remainder
-    flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
-    flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
+    // its remainder (adding a zero group changes nothing)
+    if (Egrp != 0.0) flush(S.accE, Egrp, ACC_E_HUGE, ACC_E_NAN);
+    if (Ggrp != 0.0) flush(S.accG, Ggrp, ACC_G_HUGE, ACC_G_NAN);
     return true;
 }
 
