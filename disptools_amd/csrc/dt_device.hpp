@@ -153,14 +153,10 @@ struct SumsWave {
     unsigned flags;
 };
 
-// The block's copy of accumulator slot `slot`, read from the instance record
-// at once and kept in registers (a use at the end of the pass would reload it
-// there, a round trip of its own).
+// The block's copy of accumulator slot `slot`.
 __device__ __forceinline__ DT_GLOBAL Acc *block_acc(const Inst &I, int slot)
 {
-    DT_GLOBAL Acc *a = gptr(I.acc) + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES);
-    asm volatile("" : "+s"(a));
-    return a;
+    return gptr(uni(I.acc + (long long)slot * ACC_COPIES + (blockIdx.x % ACC_COPIES)));
 }
 
 // One thread adds a block's reduced sums into its accumulator copy (relaxed,
@@ -575,8 +571,8 @@ __device__ __forceinline__ BeginLoads pass_begin_load(const Inst &I, int parity)
     static_assert(offsetof(Inst, final_ctrl) - offsetof(Inst, trace) == 24, "Inst::trace .. final_ctrl");
     const int lane = threadIdx.x & 63;
     BeginLoads L;
-    L.ctrl = I.ctrl;
-    L.acc = I.acc;
+    L.ctrl = uni(I.ctrl);
+    L.acc = uni(I.acc);
     asm volatile("" : "+s"(L.ctrl), "+s"(L.acc));
     const unsigned long long *src = (const unsigned long long *)(L.ctrl + parity);
     const unsigned long long *rec = (const unsigned long long *)(L.acc + (lane < NREC ? lane : 0));

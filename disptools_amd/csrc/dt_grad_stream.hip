@@ -398,7 +398,10 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
         // from scratch for this then waits only for loads already due.
         const bool zint = OFFZ || (zz >= 2 && zz <= nz - 3);
         ColCoef KF;
-        if (!OFFZ) KF = zint ? column_coefficients(P, gx, gy) : face_coefficients(P, gx, gy, zz);
+        if (!OFFZ) {
+            const StencilCoef cf = load_coef(I);
+            KF = zint ? column_coefficients(P, cf, gx, gy) : face_coefficients(P, cf, gx, gy, zz);
+        }
         // The loads of plane zz+2 and of this plane's target / mask were issued
         // one step ago.  An empty asm that takes them here keeps the compiler
         // from hoisting their first use above the barrier that ended the last
@@ -509,7 +512,7 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
 #pragma unroll 1
     for (; zz < z1 && zz < 2; zz += 2) trip(std::integral_constant<int, 2>{}, zz, ColCoef{});
     {
-        const ColCoef K = column_coefficients(P, gx, gy);
+        const ColCoef K = column_coefficients(P, load_coef(I), gx, gy);
         const int flo = max(z0 + 1, 2), fhi = min(z1 - 1, nz - 3);
 #pragma unroll 1
         for (; zz < z1 && zz + 1 <= nz - 3; zz += 2) {

@@ -148,7 +148,8 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
         ris[k] = rys[k] * RX + rxs[k];
     }
     const bool col_in = ins[0];
-    const ColCoef K = column_coefficients(P, gxs[0], gys[0]);
+    const StencilCoef cf = load_coef(I);
+    const ColCoef K = column_coefficients(P, cf, gxs[0], gys[0]);
     // region points on the volume's x / y outer face keep the output
     // buffer's contents in a step (the loads below skip the other lanes)
     bool xyf[2];
@@ -305,7 +306,7 @@ __device__ __forceinline__ void greedy_body(GreedySmem<TX, TY> &S, const Inst &I
                 sm[c][2][0] = col[0][c]; sm[c][2][1] = col[1][c]; sm[c][2][2] = col[3][c]; sm[c][2][3] = col[4][c];
             }
             const float dJ = (zz >= 2 && zz <= nz - 3) ? jacobian_delta_coef<CT>(K, sm, col[2])
-                                                       : jacobian_delta<CT>(P, gxs[0], gys[0], zz, sm, col[2]);
+                                                       : jacobian_delta<CT>(P, cf, gxs[0], gys[0], zz, sm, col[2]);
             const bool in_mask = cur.m != 0;
             const float e = voxel_error_delta(dJ, cur.t - 1.0f, in_mask, P.tol);
             Egrp += (double)e * (double)e;

@@ -99,6 +99,13 @@ constexpr unsigned ACC_E_NAN = 2u;
 constexpr unsigned ACC_G_HUGE = 4u;
 constexpr unsigned ACC_G_NAN = 8u;
 
+// One stencil along one axis as  d = A (p1 - m1) + B (p2 - m2) + G c  (samples
+// at -2, -1, 0, +1, +2; 1/spacing folded into the coefficients).  One row is
+// one 16-byte load.
+struct alignas(16) CoefRow {
+    float A, B, G, unused;
+};
+
 // One solve instance (a volume with its parameters and buffers).
 struct Inst {
     int nx, ny, nz;
@@ -107,6 +114,11 @@ struct Inst {
     long long n;             // voxels
     float ix, iy, iz;        // inverse spacing, (float)(1.0 / d)
     float dx, dy, dz;        // spacing (the greedy step uses it)
+    // stencil coefficients [axis][kind: 4th-order central, 2nd-order central,
+    // forward, backward], float products of the inverse spacing i formed on
+    // the host: (2/3 i, -(1/12 i), 0), (i/2, 0, 0), (2 i, -(i/2), -(3/2 i)),
+    // (2 i, -(i/2), 3/2 i)
+    CoefRow coef[3][4];
     dt_params p;
     const float *target;
     const uint8_t *mask;

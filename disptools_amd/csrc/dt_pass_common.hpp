@@ -26,8 +26,6 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, float v, unsign
 struct PassParams {
     int nx, ny, nz;
     float inv[3];     // 1 / spacing
-    float c4a[3];     // (2/3) * inv
-    float c4b[3];     // (1/12) * inv
     float c2[3];      // 0.5 * inv
     float tol, delta, zeta;
 };
@@ -38,13 +36,35 @@ __device__ __forceinline__ PassParams load_params(const Inst &I)
     P.nx = uni(I.nx); P.ny = uni(I.ny); P.nz = uni(I.nz);
     P.inv[0] = uni(I.ix); P.inv[1] = uni(I.iy); P.inv[2] = uni(I.iz);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        P.c4a[a] = (float)(2.0 / 3.0) * P.inv[a];
-        P.c4b[a] = (float)(1.0 / 12.0) * P.inv[a];
-        P.c2[a] = 0.5f * P.inv[a];
-    }
+    for (int a = 0; a < 3; ++a) P.c2[a] = uni(I.coef[a][K_C2].A);
     P.tol = uni(I.p.tolerance); P.delta = uni(I.p.delta); P.zeta = uni(I.p.zeta);
     return P;
+}
+
+// The stencil coefficient table of an instance (Inst::coef), read where a
+// step needs it: a lane loads the row of its own stencil kind (one 16-byte
+// load per axis), so a wave whose lanes hold different stencils selects
+// nothing and branches nowhere.  (The products are formed on the host: a
+// float product formed here is a vector instruction, and its wave-uniform
+// result would occupy a vector register for the whole pass.)
+struct StencilCoef {
+    const DT_GLOBAL CoefRow *t;   // [axis][kind]
+    __device__ __forceinline__ CoefRow row(int a, int kind) const
+    {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v v = *(const DT_GLOBAL f4v *)(t + 4 * a + kind);
+        return CoefRow{v.x, v.y, v.z, v.w};
+    }
+    // the 4th-order central weights, wave-uniform: (2/3) / spacing, (1/12) / spacing
+    __device__ __forceinline__ float c4a(int a) const { return uni(t[4 * a + K_C4].A); }
+    __device__ __forceinline__ float c4b(int a) const { return -uni(t[4 * a + K_C4].B); }
+};
+
+__device__ __forceinline__ StencilCoef load_coef(const Inst &I)
+{
+    StencilCoef c;
+    c.t = gptr(uni(&I.coef[0][0]));
+    return c;
 }
 
 // J - 1 at voxel (gx, gy, zz) from its samples sm[c][axis][m2, m1, p1, p2] and
@@ -52,7 +72,7 @@ __device__ __forceinline__ PassParams load_params(const Inst &I)
 // boundary lane take the 4th-order form; the others a weighted 5-point form
 // whose weights encode each lane's stencil (no divergent branches).
 template <typename CT>
-__device__ __forceinline__ float jacobian_delta(const PassParams &P, int gx, int gy, int zz,
+__device__ __forceinline__ float jacobian_delta(const PassParams &P, const StencilCoef &cf, int gx, int gy, int zz,
                                                 const float (&sm)[3][3][4], const float (&v0)[3])
 {
     CT d4[3][3];
@@ -62,8 +82,8 @@ __device__ __forceinline__ float jacobian_delta(const PassParams &P, int gx, int
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                d4[c][a] = (CT)(sm[c][a][2] - sm[c][a][1]) * (CT)P.c4a[a] -
-                           (CT)(sm[c][a][3] - sm[c][a][0]) * (CT)P.c4b[a];
+                d4[c][a] = (CT)(sm[c][a][2] - sm[c][a][1]) * (CT)cf.c4a(a) -
+                           (CT)(sm[c][a][3] - sm[c][a][0]) * (CT)cf.c4b(a);
     } else {
         int kind[3];
         classify(gx, gy, zz, P.nx, P.ny, P.nz, kind);
@@ -92,20 +112,29 @@ struct ColCoef {
     float A[3], B[3], G[3];
 };
 
-__device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, int gx, int gy)
+__device__ __forceinline__ ColCoef coef_rows(const StencilCoef &cf, const int (&kind)[3])
 {
-    int kind[3];
-    classify(gx, gy, 2, P.nx, P.ny, 5, kind);   // a plane away from the z faces
     ColCoef K;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int k = kind[a];
-        const float inv = P.inv[a];
-        K.A[a] = k == K_C4 ? P.c4a[a] : (k == K_C2 ? P.c2[a] : 2.0f * inv);
-        K.B[a] = k == K_C4 ? -P.c4b[a] : (k == K_C2 ? 0.0f : -0.5f * inv);
-        K.G[a] = k == K_FWD ? -1.5f * inv : (k == K_BWD ? 1.5f * inv : 0.0f);
+        const CoefRow r = cf.row(a, kind[a]);
+        K.A[a] = r.A; K.B[a] = r.B; K.G[a] = r.G;
     }
     return K;
+}
+
+// The region map there (classify on such a plane): a column on the outer x
+// or y face takes one-sided differences along the axes where it lies on the
+// face and 2nd-order central ones along the others; a column in the
+// penultimate layer takes 2nd-order central differences; the rest 4th-order.
+__device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, const StencilCoef &cf, int gx, int gy)
+{
+    const bool ox = gx == 0 || gx == P.nx - 1, oy = gy == 0 || gy == P.ny - 1;
+    const bool pen = gx == 1 || gx == P.nx - 2 || gy == 1 || gy == P.ny - 2;
+    const int central = (ox || oy || pen) ? K_C2 : K_C4;
+    const int kind[3] = {gx == 0 ? K_FWD : (gx == P.nx - 1 ? K_BWD : central),
+                         gy == 0 ? K_FWD : (gy == P.ny - 1 ? K_BWD : central), central};
+    return coef_rows(cf, kind);
 }
 
 // Coefficients of column (gx, gy) on a plane zz at the z faces (zz < 2 or
@@ -113,21 +142,15 @@ __device__ __forceinline__ ColCoef column_coefficients(const PassParams &P, int 
 // (src/jacobian.c:112-241): one-sided along the axes where the voxel lies on
 // the outer face, 2nd-order central along the others (on planes 1 and
 // nz - 2 off the x / y faces that is the penultimate layer's 2nd-order
-// stencil, the same thing).  The values are column_coefficients' for those
-// kinds.
-__device__ __forceinline__ ColCoef face_coefficients(const PassParams &P, int gx, int gy, int zz)
+// stencil, the same thing).
+__device__ __forceinline__ ColCoef face_coefficients(const PassParams &P, const StencilCoef &cf, int gx, int gy,
+                                                     int zz)
 {
     const int g[3] = {gx, gy, zz}, n[3] = {P.nx, P.ny, P.nz};
-    ColCoef F;
+    int kind[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const bool lo = g[a] == 0, hi = g[a] == n[a] - 1;
-        const float inv = P.inv[a];
-        F.A[a] = (lo || hi) ? 2.0f * inv : P.c2[a];
-        F.B[a] = (lo || hi) ? -0.5f * inv : 0.0f;
-        F.G[a] = lo ? -1.5f * inv : (hi ? 1.5f * inv : 0.0f);
-    }
-    return F;
+    for (int a = 0; a < 3; ++a) kind[a] = g[a] == 0 ? K_FWD : (g[a] == n[a] - 1 ? K_BWD : K_C2);
+    return coef_rows(cf, kind);
 }
 
 // J - 1 from per-lane stencil coefficients.  A lane with the 4th-order

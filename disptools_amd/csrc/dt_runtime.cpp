@@ -29,6 +29,7 @@
 #include "dt_host.hpp"
 #include "dt_internal.hpp"
 #include "dt_kernels.hpp"
+#include "dt_stencil.hpp"
 
 // ---------------------------------------------------------------------------
 // Error state (same contract as src/disptools.c:8-43)
@@ -233,6 +234,15 @@ Inst make_inst(const Plan &P, float dx, float dy, float dz, const dt_params &p, 
     h.iy = (float)(1.0 / (double)dy);
     h.iz = (float)(1.0 / (double)dz);
     h.dx = dx; h.dy = dy; h.dz = dz;
+    const float inv[3] = {h.ix, h.iy, h.iz};
+    for (int a = 0; a < 3; ++a) {
+        const float c4a = (float)(2.0 / 3.0) * inv[a], c4b = (float)(1.0 / 12.0) * inv[a];
+        const float c2 = 0.5f * inv[a], c20 = 2.0f * inv[a], c15 = 1.5f * inv[a];
+        h.coef[a][K_C4] = CoefRow{c4a, -c4b, 0.0f, 0.0f};
+        h.coef[a][K_C2] = CoefRow{c2, 0.0f, 0.0f, 0.0f};
+        h.coef[a][K_FWD] = CoefRow{c20, -c2, -c15, 0.0f};
+        h.coef[a][K_BWD] = CoefRow{c20, -c2, c15, 0.0f};
+    }
     h.p = p;
     h.target = target;
     h.mask = mask;
