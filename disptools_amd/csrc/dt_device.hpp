@@ -694,16 +694,25 @@ __device__ void copy_result_tile(const Inst &I, const Ctrl *C, const TileCoord &
     const float *__restrict__ src = I.buf[rb];
     float *__restrict__ dst = I.buf[0];
     const long long plane = (long long)I.nx * I.ny;
+    // a thread copies its column eight planes at a time: the 24 loads of a
+    // batch are all in flight before its stores
+    constexpr int B = 8;
     for (int t = threadIdx.x; t < TX * TY; t += NT) {
         const int x = tc.x0 + t % TX, y = tc.y0 + t / TX;
         if (x >= I.nx || y >= I.ny) continue;
         const long long col = (long long)y * I.nx + x;
-        for (int z = tc.z0; z < tc.z1; ++z) {
+        for (int z0 = tc.z0; z0 < tc.z1; z0 += B) {
+            float v[3][B];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const long long i = c * I.n + (long long)z * plane + col;
-                dst[i] = src[i];
-            }
+            for (int k = 0; k < B; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (z0 + k < tc.z1) v[c][k] = src[c * I.n + (long long)(z0 + k) * plane + col];
+#pragma unroll
+            for (int k = 0; k < B; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (z0 + k < tc.z1) dst[c * I.n + (long long)(z0 + k) * plane + col] = v[c][k];
         }
     }
 }
