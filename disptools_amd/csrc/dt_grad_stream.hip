@@ -412,6 +412,12 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
             if (INTERP) asm volatile("" : "+v"(rn.b[k]));
         }
         asm volatile("" : "+v"(cur.t));
+        // stage A's neighbours on plane zz (ring slot sa), read before this
+        // step's commit writes another slot: the reads then overlap the
+        // commit, the stores and the loads below
+        const f3 *R = &S.xs[sa][pi];
+        const f3 xm2 = R[-2], xm1 = R[-1], xp1 = R[1], xp2 = R[2];
+        const f3 ym2 = R[-2 * HX], ym1 = R[-HX], yp1 = R[HX], yp2 = R[2 * HX];
         {
             Val vz;
             resolve(zz + 2, rn, vz, OFFZ);
@@ -432,9 +438,6 @@ __device__ __forceinline__ bool stream_pass_body(StreamSmem<TX, TY> &S, const In
         // volume too: nothing reads their W, their error is not counted)
         CT wz_p[3] = {0, 0, 0};
         if (OFFZ || (zz >= 0 && zz < nz)) {
-            const f3 *R = &S.xs[sa][pi];
-            const f3 xm2 = R[-2], xm1 = R[-1], xp1 = R[1], xp2 = R[2];
-            const f3 ym2 = R[-2 * HX], ym1 = R[-HX], yp1 = R[HX], yp2 = R[2 * HX];
             float sm[3][3][4];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
