@@ -180,6 +180,31 @@ def test_fp64_stencil_variant(gpu_lib, oracle32, oracle64):
     assert abs(mg - mo) <= MAX_ERR_TOL
 
 
+def test_fp64_stencil_pass_variants(gpu_lib, oracle32, oracle64, monkeypatch):
+    """The all-FP64 stencil gives the same solve, bit for bit, with every
+    column tile, and holds the parity bar against the FP64 oracle.  The wide
+    tiles' FP64 builds spill registers; a miscompile there once zeroed a
+    pass's max error after the first pass (voxel-exploded stop at it 1)."""
+    T, M = sphere(24, 6)
+    fo, io = oracle64.solve("gradient", T.astype(np.float64), M, it_max=300)
+    _, mo = masked_max_error(oracle32, fo.astype(np.float32), T, M, (1.0, 1.0, 1.0))
+    monkeypatch.setenv("DISPTOOLS_STENCIL_PRECISION", "64")
+    ref = None
+    for variant in range(5):
+        monkeypatch.setenv("DISPTOOLS_GRAD_VARIANT", str(variant))
+        print("fp64 gradient variant", variant, T.shape, flush=True)
+        fg, ig, _ = gpu_lib.solve_host("gradient", T, M, it_max=300)
+        assert ig["reason"] == io["reason"], (variant, ig["reason"], io["reason"])
+        _, mg = masked_max_error(oracle32, fg, T, M, (1.0, 1.0, 1.0))
+        assert abs(mg - mo) <= MAX_ERR_TOL, variant
+        if ref is None:
+            ref = (fg, ig)
+            continue
+        np.testing.assert_array_equal(fg, ref[0])
+        for k in ("iterations", "evaluations", "rejections", "reason", "final_error", "final_max_error"):
+            assert ig[k] == ref[1][k], (variant, k)
+
+
 def test_reference_entry_points(gpu_lib, oracle32, oracle64):
     """generate_displacement_*_cuda with host buffers, as _disptools calls them."""
     import ctypes
