@@ -573,7 +573,6 @@ __device__ __forceinline__ BeginLoads pass_begin_load(const Inst &I, int parity)
     BeginLoads L;
     L.ctrl = uni(I.ctrl);
     L.acc = uni(I.acc);
-    asm volatile("" : "+s"(L.ctrl), "+s"(L.acc));
     const unsigned long long *src = (const unsigned long long *)(L.ctrl + parity);
     const unsigned long long *rec = (const unsigned long long *)(L.acc + (lane < NREC ? lane : 0));
     // lane k < W: word k of the control block; W <= k < W + VW: word k - W
