@@ -1,4 +1,5 @@
-"""Seeded random cases against the oracle: shapes 3..41 per axis, anisotropic
+"""Seeded random cases against the oracle: shapes 3..41 per axis (and a few of
+30..90, which span several z-chunks and column tiles), anisotropic
 spacing, random masks and targets, and random solver parameters (alpha,
 beta, eta, strict, it_max), gradient and greedy.  Bar: the stop reason of
 the FP64 or the FP32 oracle (they may disagree at rounding level), a final
@@ -10,23 +11,24 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _cases(seed=7, count=30):
+def _cases(seed=7, count=30, sides=(3, 42), it_max=(20, 200)):
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
-        shape = tuple(int(v) for v in rng.integers(3, 42, size=3))
+        shape = tuple(int(v) for v in rng.integers(sides[0], sides[1], size=3))
         alg = ["gradient", "greedy"][int(rng.integers(0, 2))]
         sp = tuple(float(v) for v in rng.uniform(0.6, 1.6, size=3))
         T = (1.0 + rng.uniform(0.02, 0.12) * rng.standard_normal(shape)).astype(np.float32)
         M = rng.random(shape) < rng.uniform(0.3, 1.0)
-        params = dict(it_max=int(rng.integers(20, 200)), alpha=float(rng.uniform(1.05, 1.5)),
+        params = dict(it_max=int(rng.integers(it_max[0], it_max[1])), alpha=float(rng.uniform(1.05, 1.5)),
                       beta=float(rng.uniform(0.3, 0.7)), eta=float(rng.uniform(0.05, 0.3)),
                       strict=bool(rng.random() < 0.3))
         out.append((alg, T, M, sp, params))
     return out
 
 
-CASES = _cases()
+# the larger volumes keep it_max low so the CPU oracle stays quick
+CASES = _cases() + _cases(seed=21, count=8, sides=(30, 91), it_max=(10, 60))
 
 
 @pytest.mark.parametrize("k", range(len(CASES)))
