@@ -134,6 +134,36 @@ def test_engine_equals_plain_solve_for_any_tiling(gpu_lib, tz):
         np.testing.assert_array_equal(ef[i], f)
 
 
+def _random_batch(seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        shape = tuple(int(v) for v in rng.integers(3, 91, size=3))
+        alg = ["gradient", "greedy"][int(rng.integers(0, 2))]
+        sp = tuple(float(v) for v in rng.uniform(0.6, 1.6, size=3))
+        T = (1.0 + rng.uniform(0.02, 0.12) * rng.standard_normal(shape)).astype(np.float32)
+        M = rng.random(shape) < rng.uniform(0.3, 1.0)
+        out.append((alg, T, M, sp, int(rng.integers(10, 120))))
+    return out
+
+
+def test_engine_random_batch_equals_lone_solves(gpu_lib):
+    """Sixteen seeded volumes of 3..90 voxels per axis, gradient and greedy,
+    random spacing, masks and iteration limits, admitted five at once and the
+    rest as slots free: every result equals its lone dt_solve_host, bit for
+    bit."""
+    cases = _random_batch(31, 16)
+    with gpu_lib.Engine() as eng:
+        ef, er = _run_all(eng, cases, stagger=5)
+    for i, (alg, T, M, sp, it_max) in enumerate(cases):
+        print("random batch", i, alg, T.shape, flush=True)
+        f, info, _ = gpu_lib.solve_host(alg, T, M, sp, it_max=it_max)
+        assert er[i][0] == 0
+        for k in SAME_KEYS:
+            assert er[i][1][k] == info[k], (i, k, er[i][1][k], info[k])
+        np.testing.assert_array_equal(ef[i], f)
+
+
 def test_engine_parity_with_fp64_oracle(gpu_lib, oracle32, oracle64):
     cases = _cases()
     with gpu_lib.Engine() as eng:
