@@ -149,6 +149,20 @@ __device__ __forceinline__ void jac_step(const JacGeom &G, const JacLane &T, flo
             d[c][2] = dstencil_c4<double>(q[c][S % JQ], q[c][(S + 1) % JQ], q[c][(S + 3) % JQ],
                                           q[c][(S + 4) % JQ], G.iz);
         }
+    } else if (__builtin_amdgcn_ballot_w64(kind[1] > K_C2 || kind[2] > K_C2) == 0) {
+        // along y and z every lane's stencil is central (a patch at the x
+        // faces, away from the others): the general form along x only
+        const StencilSel sx = stencil_sel(kind[0]);
+        const bool cy = kind[1] == K_C4, cz = kind[2] == K_C4;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float *row = P + c * WPLANE + (T.ly + 2) * HX + T.lx;
+            const float *clm = P + c * WPLANE + T.ly * HX + T.lx + 2;
+            d[c][0] = dstencil_sel(sx, row[0], row[1], row[2], row[3], row[4], G.ix);
+            d[c][1] = dstencil_c4c2(cy, clm[0], clm[HX], clm[3 * HX], clm[4 * HX], G.iy);
+            d[c][2] = dstencil_c4c2(cz, q[c][S % JQ], q[c][(S + 1) % JQ], q[c][(S + 3) % JQ], q[c][(S + 4) % JQ],
+                                    G.iz);
+        }
     } else {
         const StencilSel sx = stencil_sel(kind[0]), sy = stencil_sel(kind[1]), sz = stencil_sel(kind[2]);
 #pragma unroll

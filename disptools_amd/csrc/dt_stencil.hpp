@@ -67,6 +67,15 @@ __host__ __device__ __forceinline__ CT dstencil_c4(float m2, float m1, float p1,
     return ((CT)(p1 - m1) * (CT)(2.0 / 3.0) - (CT)(p2 - m2) * (CT)(1.0 / 12.0)) * (CT)inv;
 }
 
+// dstencil<double> for lanes whose stencil along the axis is 4th- or
+// 2nd-order central (c4 picks): both forms rounded as dstencil's, one kept.
+__host__ __device__ __forceinline__ double dstencil_c4c2(bool c4, float m2, float m1, float p1, float p2, float inv)
+{
+    const double a = dstencil_c4<double>(m2, m1, p1, p2, inv);
+    const double b = (double)((p1 - m1) * inv) * 0.5;
+    return c4 ? a : b;
+}
+
 // dstencil<double> without branches, for lanes of one wave that hold
 // different stencil kinds.  Every kind is evaluated with the operations of
 // dstencil in the same order -- a - b*w as a + b*(-w), which rounds the same
